@@ -1,0 +1,180 @@
+/*
+ * mpm_hip.h -- C ABI of libmpm_hip.so, the MI355X (gfx950) implementation of
+ * MaterialPointSolver.jl's explicit single-phase MUSL timestep.
+ *
+ * The library replaces the reference's device plugin for this path
+ * (ext/AMDExt/devicehelpfunc_amd.jl plus the KernelAbstractions kernels that
+ * procedure!(..., Val(:OS), Val(:MUSL), Val(:a|:v)) launches).  Each entry
+ * point below names the reference interface it stands in for.  A Julia
+ * package extension binds these with ccall (a sketch ships in the package's
+ * julia/ folder); Python binds them with ctypes.
+ *
+ * Conventions
+ *  - Every call returns MPM_OK (0) or a negative MPM_E_* code; the message of
+ *    the last failure is kept per context (mpm_last_error).
+ *  - Host arrays are the reference's Julia layouts: column-major (rows, k)
+ *    matrices, i.e. k contiguous streams of `rows` values, in FP64 / Int64.
+ *    Indices (nid, BC flags) keep the reference's 1-based meaning.
+ *  - Grid-node arrays (BC) use the reference's node order: 2D rows from the
+ *    top (y descending, src/types/grid.jl:92-100), 3D y fastest then x then z
+ *    (src/types/grid.jl:236-249).
+ *  - The library never keeps a host pointer after a call returns.
+ *  - A context is driven from one thread; contexts are independent.
+ *  - Step calls are asynchronous on the context's own HIP stream; download,
+ *    mpm_sync and mpm_destroy synchronise.
+ */
+#ifndef MPM_HIP_H
+#define MPM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_ABI_VERSION 1
+
+enum {
+    MPM_OK = 0,
+    MPM_E_ARG = -1,         /* invalid argument or configuration */
+    MPM_E_UNSUPPORTED = -2, /* a switch value this library does not implement */
+    MPM_E_HIP = -3,         /* HIP runtime error (message carries hipGetErrorString) */
+    MPM_E_NOMEM = -4,       /* device allocation failed */
+    MPM_E_STATE = -5,       /* call out of order (e.g. step before upload) */
+    MPM_E_RANGE = -6        /* a particle left the grid or became non-finite */
+};
+
+enum { MPM_FP64 = 0, MPM_FP32 = 1 };                 /* precision */
+enum { MPM_UGIMP = 0, MPM_LINEAR = 1 };               /* basis */
+enum { MPM_MUSL = 0 };                                /* scheme */
+enum { MPM_VA_A = 0, MPM_VA_V = 1 };                  /* va */
+enum { MPM_LINEARELASTIC = 0, MPM_DRUCKERPRAGER = 1, MPM_MOHRCOULOMB = 2 }; /* constitutive */
+
+/* Device-side model description, fixed for the life of a context: the
+ * scalars of Grid2D/Grid3D and Particle2D/Particle3D the step reads
+ * (src/types/grid.jl:33-75,170-217; src/types/particle.jl:33-81,160-210) plus
+ * the precision chosen by the structs' T2 (the reference's ϵ keyword). */
+typedef struct mpm_config {
+    int32_t dim;           /* 2 | 3 */
+    int32_t precision;     /* MPM_FP64: every field FP64 (parity mode).
+                              MPM_FP32: FP32 bulk arithmetic and grid atomics,
+                              FP64 particle positions and det F. */
+    int32_t basis;         /* MPM_UGIMP (grid.NIC 9 | 27) or MPM_LINEAR (NIC 4 | 8) */
+    int32_t sort_interval; /* steps between on-device cell sorts; 0 = never */
+    int64_t np;            /* number of particles (mp.np) */
+    int64_t nmat;          /* number of materials (length of the property vectors) */
+    int64_t nnx, nny, nnz; /* nodes per axis (grid.nnx, nny, nnz; nnz = 1 in 2D) */
+    double x1, y1, z1;     /* grid origin (grid.x1, y1, z1) */
+    double dx, dy, dz;     /* grid spacing (grid.dx, dy, dz) */
+    double lpx, lpy, lpz;  /* particle spacing (mp.dx, dy, dz), uGIMP domain size */
+} mpm_config;
+
+/* Per-call switches: the fields of Args2D/Args3D that procedure! reads
+ * (src/types/modelargs.jl:31-58,122-149; src/solvers/OS_MUSL.jl:12-102). */
+typedef struct mpm_args {
+    int32_t scheme;        /* MPM_MUSL only (args.scheme) */
+    int32_t va;            /* MPM_VA_A | MPM_VA_V (args.va) */
+    int32_t constitutive;  /* MPM_LINEARELASTIC | MPM_DRUCKERPRAGER | MPM_MOHRCOULOMB */
+    int32_t mvl;           /* args.MVL: cell-averaged mean stress */
+    double flip, pic;      /* args.FLIP, args.PIC */
+    double zeta_s;         /* args.ζs, solid damping */
+    double gravity;        /* args.gravity, on the last axis */
+    double Te;             /* args.Te: gravity ramp end and plasticity gate */
+} mpm_args;
+
+typedef struct mpm_ctx mpm_ctx;
+
+/* ABI version of the loaded library (MPM_ABI_VERSION). */
+int mpm_abi_version(void);
+
+/* Create a context on the current HIP device: validates cfg, allocates every
+ * device buffer the step needs and zeroes the grid.
+ * Replaces the allocation half of host2device(grid, mp, attr, bc, Val(:ROCm))
+ * (ext/AMDExt/devicehelpfunc_amd.jl:17-37). */
+int mpm_create(const mpm_config *cfg, mpm_ctx **out);
+
+/* Upload particle state (np rows each).  xi, vs: np x dim; sigma, eps_s:
+ * np x (4|6) in the reference's Voigt order ([xx,yy,zz,xy] 2D,
+ * [xx,yy,zz,xy,yz,xz] 3D); F: np x (4|9) row-major tensor per particle;
+ * omega, omega0, rho, rho0, eps_q, eps_k: np; nid: np, 1-based material ids.
+ * Replaces the particle half of host2device (devicehelpfunc_amd.jl:26). */
+int mpm_upload_particles(mpm_ctx *ctx, const double *xi, const double *vs,
+                         const double *sigma, const double *eps_s, const double *F,
+                         const double *omega, const double *omega0, const double *rho,
+                         const double *rho0, const double *eps_q, const double *eps_k,
+                         const int64_t *nid);
+
+/* Upload the material table (nmat entries each; attr.Ks, Gs, ϕ, ψ, c, σt, Hp, cr;
+ * src/types/property.jl:31-52).  Replaces host2device for attr
+ * (devicehelpfunc_amd.jl:27). */
+int mpm_upload_materials(mpm_ctx *ctx, const double *Ks, const double *Gs,
+                         const double *phi, const double *psi, const double *c,
+                         const double *sigma_t, const double *Hp, const double *cr);
+
+/* Upload the solid velocity boundary (bc.v{x,y,z}_s_idx / _val, ni entries each in
+ * the reference's node order; vz_* ignored in 2D; src/types/boundary.jl:115-131,169-190).
+ * Replaces host2device for bc (devicehelpfunc_amd.jl:28). */
+int mpm_upload_bc(mpm_ctx *ctx, const int64_t *vx_idx, const double *vx_val,
+                  const int64_t *vy_idx, const double *vy_val,
+                  const int64_t *vz_idx, const double *vz_val);
+
+/* One timestep at time Ti: the whole of
+ * procedure!(args, grid, mp, attr, bc, ΔT, Ti, Val(:OS), Val(:MUSL), Val(args.va))
+ * (src/solvers/OS_MUSL.jl:12-102): gravity ramp Ti < Te, plasticity when
+ * Ti >= Te.  Asynchronous. */
+int mpm_step(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti);
+
+/* nsteps timesteps with the submit_work! clock: Ti += dt after each step,
+ * accumulated in FP64 (src/solver.jl:118-124).  *Ti_out receives the final
+ * time (may be NULL).  Asynchronous. */
+int mpm_run(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int64_t nsteps,
+            double *Ti_out);
+
+/* Wait for all work on the context's stream; reports range/finite flags the
+ * kernels raised.  Replaces KAsync(getBackend(Val(:ROCm))) (src/solver.jl:128). */
+int mpm_sync(mpm_ctx *ctx);
+
+/* Download the device2host! field set, in the caller's original particle order
+ * (ext/AMDExt/devicehelpfunc_amd.jl:39-69): xi, vs (np x dim), sigma, eps_s
+ * (np x (4|6)), sigma_m, omega, eps_q, eps_k, eps_v, ms (np).  Any pointer may
+ * be NULL to skip that field.  Synchronises. */
+int mpm_download_particles(mpm_ctx *ctx, double *xi, double *vs, double *sigma,
+                           double *sigma_m, double *omega, double *eps_q, double *eps_k,
+                           double *eps_v, double *eps_s, double *ms);
+
+/* Free every device buffer of the context and the context itself.
+ * Replaces clean_device!(..., Val(:ROCm)) (devicehelpfunc_amd.jl:71-102). */
+void mpm_destroy(mpm_ctx *ctx);
+
+/* Message of the last failed call on ctx (or of the last failed mpm_create
+ * when ctx is NULL).  Never NULL. */
+const char *mpm_last_error(const mpm_ctx *ctx);
+
+/* Per-kernel timing with HIP events on the context's stream.  Enable before
+ * the steps to measure; mpm_profile_read fills up to n entries of the kernel
+ * names' order given by mpm_profile_kernel_name: total milliseconds and
+ * launch counts since the last enable.  Synchronises.
+ * No reference counterpart (the reference's perf() reports only it/s,
+ * src/toolkits/terminaltxt.jl:67-85). */
+int mpm_profile_enable(mpm_ctx *ctx, int on);
+int mpm_profile_read(mpm_ctx *ctx, double *total_ms, int64_t *launches, int n);
+const char *mpm_profile_kernel_name(int k);
+int mpm_profile_kernel_count(void);
+
+/* Number of grid nodes that receive a non-zero basis weight from at least one
+ * particle at the current positions (the "active" nodes whose traffic the
+ * step's algorithmic byte count includes).  Diagnostic; synchronises.
+ * No reference counterpart. */
+int mpm_count_active_nodes(mpm_ctx *ctx, int64_t *count);
+
+/* Triad bandwidth probe A = B + C on nbytes-per-array FP32 buffers, in GB/s
+ * (10^9 B/s), best of `reps`.  Replaces Tpeak(Val(:ROCm))
+ * (ext/AMDExt/devicehelpfunc_amd.jl:104-142), which reports GiB/s of the same
+ * triad. */
+int mpm_tpeak(int64_t nbytes, int reps, double *gbps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPM_HIP_H */

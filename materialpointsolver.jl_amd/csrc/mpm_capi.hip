@@ -1,0 +1,796 @@
+// mpm_capi.hip -- C ABI of libmpm_hip.so (declared in include/mpm_hip.h).
+//
+// Host side of the gfx950 MUSL step: context and device buffers, uploads from
+// the reference's Julia layouts, the per-step launch sequence, downloads in
+// the caller's particle order, HIP-event kernel timing and the triad probe.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mpm_hip.h"
+#include "mpm_kernels.hpp"
+#include "mpm_sort.hpp"
+
+using namespace mpm;
+
+namespace {
+
+std::string g_create_error = "no error";
+
+const char* const kKernelNames[] = {"k_p2g", "k_grid1", "k_g2p1_p2g2", "k_grid2",
+                                    "k_g2p2", "k_mvl2", "k_sort", "k_permute"};
+constexpr int kNumKernels = int(sizeof(kKernelNames) / sizeof(kKernelNames[0]));
+enum { KID_P2G = 0, KID_GRID1, KID_G2P1, KID_GRID2, KID_G2P2, KID_MVL2, KID_SORT, KID_PERMUTE };
+
+constexpr int kBlock = 256;
+
+inline unsigned nblocks(long long n) { return unsigned((n + kBlock - 1) / kBlock); }
+
+// Julia's Base.min for IEEE floats (NaN propagates).
+inline double jl_min(double x, double y) {
+    if (std::isnan(x)) return x;
+    if (std::isnan(y)) return y;
+    if (y < x || (std::signbit(y) && !std::signbit(x))) return y;
+    return x;
+}
+
+struct ProfRec {
+    int kid;
+    hipEvent_t e0, e1;
+};
+
+}  // namespace
+
+struct ImplBase;
+
+struct mpm_ctx {
+    mpm_config cfg{};
+    hipStream_t stream = nullptr;
+    std::string err = "no error";
+    ImplBase* impl = nullptr;
+    int* d_flags = nullptr;
+    bool have_particles = false, have_materials = false, have_bc = false;
+    long long steps = 0;
+    bool prof_on = false;
+    std::vector<ProfRec> recs;
+    std::vector<hipEvent_t> pool;
+
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+    int hip_fail(hipError_t e, const char* what) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return MPM_E_HIP;
+    }
+    hipEvent_t event() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    // Launch wrapper: optional event pair around the launch on the context stream.
+    template <typename Fn>
+    int launch(int kid, Fn&& fn) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (prof_on) {
+            e0 = event();
+            if (e0) hipEventRecord(e0, stream);
+        }
+        fn();
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, kKernelNames[kid]);
+        if (prof_on && e0) {
+            e1 = event();
+            if (e1) {
+                hipEventRecord(e1, stream);
+                recs.push_back({kid, e0, e1});
+            }
+        }
+        return MPM_OK;
+    }
+};
+
+struct ImplBase {
+    virtual ~ImplBase() {}
+    virtual int upload_particles(mpm_ctx*, const double*, const double*, const double*,
+                                 const double*, const double*, const double*, const double*,
+                                 const double*, const double*, const double*, const double*,
+                                 const int64_t*) = 0;
+    virtual int upload_materials(mpm_ctx*, const double*, const double*, const double*,
+                                 const double*, const double*, const double*, const double*,
+                                 const double*) = 0;
+    virtual int upload_bc(mpm_ctx*, const int64_t*, const double*, const int64_t*, const double*,
+                          const int64_t*, const double*) = 0;
+    virtual int step(mpm_ctx*, const mpm_args& a, double dt, double Ti) = 0;
+    virtual int download(mpm_ctx*, double*, double*, double*, double*, double*, double*,
+                         double*, double*, double*, double*) = 0;
+    virtual int count_active(mpm_ctx*, int64_t*) = 0;
+};
+
+namespace {
+
+template <typename T>
+hipError_t dalloc(T** p, size_t n) {
+    return hipMalloc((void**)p, sizeof(T) * (n ? n : 1));
+}
+
+// dst[pid[i]] = double(src[i])
+template <typename T>
+__global__ __launch_bounds__(256) void k_unpermute(const T* __restrict__ src,
+                                                   const int* __restrict__ pid,
+                                                   double* __restrict__ dst, int np) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) dst[pid[i]] = double(src[i]);
+}
+
+// σm = (σxx + σyy + σzz) * 0.333333 of the stored stress (liE!/dpP!/mcP! all
+// end with this assignment, linearelastic.jl:61-62), scattered to caller order.
+template <typename R>
+__global__ __launch_bounds__(256) void k_sigm(const R* s0, const R* s1, const R* s2,
+                                              const int* __restrict__ pid,
+                                              double* __restrict__ dst, int np) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) dst[pid[i]] = double((s0[i] + s1[i] + s2[i]) * R(0.333333));
+}
+
+__global__ __launch_bounds__(256) void k_triad(const float4* __restrict__ b,
+                                               const float4* __restrict__ c,
+                                               float4* __restrict__ a, long long n4) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n4; i += stride) {
+        float4 x = b[i], y = c[i];
+        a[i] = make_float4(x.x + y.x, x.y + y.y, x.z + y.z, x.w + y.w);
+    }
+}
+
+template <typename R>
+struct Impl : ImplBase {
+    int dim, ns, nf;
+    long long np, ni, nc;
+    GridGeom geom{};
+    PartPtrs<R> pp{};
+    GridPtrs<R> gp{};
+    Material<R>* d_mats = nullptr;
+    double* d_stage = nullptr;  // np doubles, download staging
+    std::vector<void*> owned;
+    SortState<R> sort;
+
+    ~Impl() override {
+        for (void* p : owned) hipFree(p);
+        sort.release();
+    }
+
+    template <typename T>
+    int alloc(mpm_ctx* c, T** p, size_t n, const char* what) {
+        hipError_t e = dalloc(p, n);
+        if (e != hipSuccess) return c->fail(MPM_E_NOMEM, std::string("hipMalloc ") + what + ": " +
+                                                             hipGetErrorString(e));
+        owned.push_back((void*)*p);
+        return MPM_OK;
+    }
+
+    int init(mpm_ctx* c) {
+        const mpm_config& cf = c->cfg;
+        dim = cf.dim;
+        ns = dim == 2 ? 4 : 6;
+        nf = dim * dim;
+        np = cf.np;
+        geom.nn[0] = int(cf.nnx);
+        geom.nn[1] = int(cf.nny);
+        geom.nn[2] = dim == 3 ? int(cf.nnz) : 1;
+        geom.nc[0] = geom.nn[0] - 1;
+        geom.nc[1] = geom.nn[1] - 1;
+        geom.nc[2] = dim == 3 ? geom.nn[2] - 1 : 1;
+        ni = (long long)geom.nn[0] * geom.nn[1] * geom.nn[2];
+        nc = (long long)geom.nc[0] * geom.nc[1] * geom.nc[2];
+        geom.ni = ni;
+        const double org[3] = {cf.x1, cf.y1, cf.z1}, h[3] = {cf.dx, cf.dy, cf.dz},
+                     lp[3] = {cf.lpx, cf.lpy, cf.lpz};
+        for (int a = 0; a < 3; ++a) {
+            geom.org[a] = org[a];
+            geom.h[a] = h[a];
+            geom.hinv[a] = 1.0 / h[a];
+            geom.lp[a] = lp[a];
+            geom.c4[a] = 1.0 / (4.0 * h[a] * lp[a]);
+            geom.c1[a] = 1.0 / (h[a] * lp[a]);
+            geom.half[a] = 0.5 * lp[a];
+        }
+        int rc;
+#define A(ptr, n, what) \
+    if ((rc = alloc(c, &(ptr), (n), what)) != MPM_OK) return rc;
+        for (int a = 0; a < dim; ++a) {
+            A(pp.x[a], np, "x");
+            A(pp.v[a], np, "v");
+            A(pp.dxp[a], np, "dx");
+        }
+        for (int k = 0; k < ns; ++k) {
+            A(pp.sig[k], np, "sigma");
+            A(pp.eps[k], np, "eps");
+        }
+        A(pp.ms, np, "ms");
+        A(pp.vol, np, "vol");
+        A(pp.vol0, np, "vol0");
+        A(pp.J, np, "J");
+        A(pp.eq, np, "eq");
+        A(pp.ek, np, "ek");
+        A(pp.ev, np, "ev");
+        A(pp.nid, np, "nid");
+        A(pp.pid, np, "pid");
+        A(pp.p2c, np, "p2c");
+        A(gp.m, ni, "grid m");
+        for (int d = 0; d < dim; ++d) {
+            A(gp.p[d], ni, "grid p");
+            A(gp.f[d], ni, "grid f");
+            A(gp.vs[d], ni, "grid vs");
+            A(gp.vsT[d], ni, "grid vsT");
+            A(gp.as[d], ni, "grid as");
+            A(gp.du[d], ni, "grid du");
+            A(gp.bcv[d], ni, "grid bc value");
+        }
+        A(gp.bcf, ni, "grid bc flags");
+        A(gp.cell_sm, nc, "cell sm");
+        A(gp.cell_vol, nc, "cell vol");
+        A(d_stage, np, "staging");
+        A(d_mats, cf.nmat, "materials");
+#undef A
+        // zero every grid field once; afterwards each step leaves m, p, f at zero
+        hipError_t e = hipSuccess;
+        e = hipMemset(gp.m, 0, sizeof(R) * ni);
+        for (int d = 0; d < dim && e == hipSuccess; ++d) {
+            e = hipMemset(gp.p[d], 0, sizeof(R) * ni);
+            if (e == hipSuccess) e = hipMemset(gp.f[d], 0, sizeof(R) * ni);
+            if (e == hipSuccess) e = hipMemset(gp.bcv[d], 0, sizeof(R) * ni);
+            if (e == hipSuccess) e = hipMemset(gp.du[d], 0, sizeof(R) * ni);
+        }
+        if (e == hipSuccess) e = hipMemset(gp.bcf, 0, ni);
+        if (e == hipSuccess) e = hipMemset(gp.cell_sm, 0, sizeof(R) * nc);
+        if (e == hipSuccess) e = hipMemset(gp.cell_vol, 0, sizeof(R) * nc);
+        if (e != hipSuccess) return c->hip_fail(e, "hipMemset");
+        if ((rc = sort.init(c->cfg, geom, np)) != MPM_OK)
+            return c->fail(rc, "sort buffers: " + sort.error);
+        if (sort.enabled) {
+            char* sR = nullptr;
+            if ((rc = alloc(c, &sR, sizeof(R) * np, "sort scratch")) != MPM_OK) return rc;
+            sort.scratchR = sR;
+            char* sx = nullptr;
+            if ((rc = alloc(c, &sx, (sizeof(R) == 4 ? 8 : 4) * np, "sort scratch")) != MPM_OK)
+                return rc;
+            if (sizeof(R) == 4) sort.scratch8 = sx;
+            else sort.scratch4 = sx;
+        }
+        return MPM_OK;
+    }
+
+    int put(mpm_ctx* c, R* dst, const double* src, const char* what) {
+        std::vector<R> tmp(np);
+        for (long long i = 0; i < np; ++i) tmp[i] = R(src[i]);
+        hipError_t e = hipMemcpy(dst, tmp.data(), sizeof(R) * np, hipMemcpyHostToDevice);
+        return e == hipSuccess ? MPM_OK : c->hip_fail(e, what);
+    }
+
+    int upload_particles(mpm_ctx* c, const double* xi, const double* vs, const double* sigma,
+                         const double* eps_s, const double* F, const double* omega,
+                         const double* omega0, const double* rho, const double* rho0,
+                         const double* eps_q, const double* eps_k, const int64_t* nid) override {
+        if (!xi || !vs || !sigma || !eps_s || !F || !omega || !omega0 || !rho || !rho0 ||
+            !eps_q || !eps_k || !nid)
+            return c->fail(MPM_E_ARG, "mpm_upload_particles: NULL array");
+        int rc;
+        hipError_t e;
+        for (int a = 0; a < dim; ++a) {
+            e = hipMemcpy(pp.x[a], xi + a * np, sizeof(double) * np, hipMemcpyHostToDevice);
+            if (e != hipSuccess) return c->hip_fail(e, "upload xi");
+            if ((rc = put(c, pp.v[a], vs + a * np, "upload vs")) != MPM_OK) return rc;
+        }
+        for (int k = 0; k < ns; ++k) {
+            if ((rc = put(c, pp.sig[k], sigma + k * np, "upload sigma")) != MPM_OK) return rc;
+            if ((rc = put(c, pp.eps[k], eps_s + k * np, "upload eps_s")) != MPM_OK) return rc;
+        }
+        std::vector<double> J(np), ms(np);
+        std::vector<int> nid32(np), pid(np);
+        for (long long i = 0; i < np; ++i) {
+            const double* f = F + i;
+            auto Fk = [&](int k) { return f[(long long)k * np]; };
+            if (dim == 2) {
+                J[i] = Fk(0) * Fk(3) - Fk(1) * Fk(2);
+            } else {
+                J[i] = Fk(0) * Fk(4) * Fk(8) + Fk(1) * Fk(5) * Fk(6) + Fk(2) * Fk(3) * Fk(7) -
+                       Fk(6) * Fk(4) * Fk(2) - Fk(7) * Fk(5) * Fk(0) - Fk(8) * Fk(3) * Fk(1);
+            }
+            ms[i] = omega[i] * rho[i];
+            if (nid[i] < 1 || nid[i] > c->cfg.nmat)
+                return c->fail(MPM_E_ARG, "mpm_upload_particles: nid[" + std::to_string(i) +
+                                              "] = " + std::to_string(nid[i]) +
+                                              " outside 1.." + std::to_string(c->cfg.nmat));
+            nid32[i] = int(nid[i] - 1);
+            pid[i] = int(i);
+            if (!(std::isfinite(xi[i]) && std::isfinite(xi[np + i]) &&
+                  (dim == 2 || std::isfinite(xi[2 * np + i]))))
+                return c->fail(MPM_E_RANGE, "mpm_upload_particles: particle " + std::to_string(i + 1) +
+                                                " has a non-finite position");
+        }
+        e = hipMemcpy(pp.J, J.data(), sizeof(double) * np, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return c->hip_fail(e, "upload J");
+        if ((rc = put(c, pp.ms, ms.data(), "upload ms")) != MPM_OK) return rc;
+        if ((rc = put(c, pp.vol, omega, "upload omega")) != MPM_OK) return rc;
+        if ((rc = put(c, pp.vol0, omega0, "upload omega0")) != MPM_OK) return rc;
+        if ((rc = put(c, pp.eq, eps_q, "upload eps_q")) != MPM_OK) return rc;
+        if ((rc = put(c, pp.ek, eps_k, "upload eps_k")) != MPM_OK) return rc;
+        e = hipMemcpy(pp.nid, nid32.data(), sizeof(int) * np, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(pp.pid, pid.data(), sizeof(int) * np, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(pp.ev, 0, sizeof(R) * np);
+        if (e == hipSuccess) e = hipMemset(pp.p2c, 0xff, sizeof(int) * np);  // -1: no cell
+        for (int a = 0; a < dim && e == hipSuccess; ++a) e = hipMemset(pp.dxp[a], 0, sizeof(R) * np);
+        if (e != hipSuccess) return c->hip_fail(e, "upload nid/pid");
+        return MPM_OK;
+    }
+
+    int upload_materials(mpm_ctx* c, const double* Ks, const double* Gs, const double* phi,
+                         const double* psi, const double* coh, const double* sigt,
+                         const double* Hp, const double* cr) override {
+        if (!Ks || !Gs || !phi || !psi || !coh || !sigt || !Hp || !cr)
+            return c->fail(MPM_E_ARG, "mpm_upload_materials: NULL array");
+        const long long nm = c->cfg.nmat;
+        std::vector<Material<R>> M(nm);
+        for (long long k = 0; k < nm; ++k) {
+            // druckerprager.jl:178-183, same expressions in FP64
+            double kphi = (6.0 * coh[k] * std::cos(phi[k])) / (1.732051 * (3.0 + std::sin(phi[k])));
+            double qphi = (6.0 * std::sin(phi[k])) / (1.732051 * (3.0 + std::sin(phi[k])));
+            double qpsi = (6.0 * std::sin(psi[k])) / (1.732051 * (3.0 + std::sin(psi[k])));
+            double st = jl_min(sigt[k], kphi / qphi);
+            double ab = std::sqrt(1.0 + qphi * qphi) - qphi;
+            double tb = kphi - qphi * st;
+            Material<R>& m = M[k];
+            m.Ks = R(Ks[k]);
+            m.Gs = R(Gs[k]);
+            m.Dt = R(Ks[k] + 1.333333 * Gs[k]);
+            m.Dd = R(Ks[k] - 0.666667 * Gs[k]);
+            m.kphi = R(kphi);
+            m.qphi = R(qphi);
+            m.qpsi = R(qpsi);
+            m.sigt = R(st);
+            m.ab = R(ab);
+            m.tb = R(tb);
+            m.eq_shear = R(std::sqrt(0.333333 + 0.222222 * qpsi * qpsi));
+            m.sinphi = R(std::sin(phi[k]));
+            m.cosphi = R(std::cos(phi[k]));
+            m.tanphi = R(std::tan(phi[k]));
+            m.c = R(coh[k]);
+            m.Hp = R(Hp[k]);
+            m.cr = R(cr[k]);
+        }
+        hipError_t e = hipMemcpy(d_mats, M.data(), sizeof(Material<R>) * nm, hipMemcpyHostToDevice);
+        return e == hipSuccess ? MPM_OK : c->hip_fail(e, "upload materials");
+    }
+
+    // Reference node id (0-based) -> this library's x-fastest id.
+    long long node_map(long long n) const {
+        const long long nnx = geom.nn[0], nny = geom.nn[1];
+        if (dim == 2) {
+            long long r = n % nny, col = n / nny;  // r: row from the top
+            long long iy = nny - 1 - r;
+            return col + nnx * iy;
+        }
+        long long iy = n % nny, rest = n / nny;
+        long long ix = rest % nnx, iz = rest / nnx;
+        return ix + nnx * (iy + nny * iz);
+    }
+
+    int upload_bc(mpm_ctx* c, const int64_t* vxi, const double* vxv, const int64_t* vyi,
+                  const double* vyv, const int64_t* vzi, const double* vzv) override {
+        const int64_t* idx[3] = {vxi, vyi, vzi};
+        const double* val[3] = {vxv, vyv, vzv};
+        for (int d = 0; d < dim; ++d)
+            if (!idx[d] || !val[d]) return c->fail(MPM_E_ARG, "mpm_upload_bc: NULL array");
+        std::vector<unsigned char> flags(ni, 0);
+        std::vector<R> v[3];
+        for (int d = 0; d < dim; ++d) v[d].assign(ni, R(0));
+        for (long long n = 0; n < ni; ++n) {
+            long long q = node_map(n);
+            for (int d = 0; d < dim; ++d) {
+                if (idx[d][n] != 0) flags[q] |= (unsigned char)(1u << d);
+                v[d][q] = R(val[d][n]);
+            }
+        }
+        hipError_t e = hipMemcpy(gp.bcf, flags.data(), ni, hipMemcpyHostToDevice);
+        for (int d = 0; d < dim && e == hipSuccess; ++d)
+            e = hipMemcpy(gp.bcv[d], v[d].data(), sizeof(R) * ni, hipMemcpyHostToDevice);
+        return e == hipSuccess ? MPM_OK : c->hip_fail(e, "upload bc");
+    }
+
+    template <int DIM, int S>
+    int step_impl(mpm_ctx* c, const mpm_args& ar, double dt, double Ti) {
+        const mpm_config& cf = c->cfg;
+        // OS_MUSL.jl:70 gravity ramp and :88 plasticity gate, evaluated in FP64
+        double G = Ti < ar.Te ? ar.gravity / ar.Te * Ti : ar.gravity;
+        StepArgs<R> sa;
+        sa.dt = R(dt);
+        sa.G = R(G);
+        sa.flip = R(ar.flip);
+        sa.pic = R(ar.pic);
+        sa.zeta = R(ar.zeta_s);
+        sa.va = ar.va;
+        sa.plastic = Ti >= ar.Te ? 1 : 0;
+        sa.constitutive = ar.constitutive;
+        sa.mvl = ar.mvl;
+        const int n = int(np);
+        const unsigned pb = nblocks(np), gb = nblocks(ni > nc ? ni : nc);
+        hipStream_t s = c->stream;
+        int* fl = c->d_flags;
+        int rc;
+        if (cf.sort_interval > 0 && (c->steps % cf.sort_interval) == 0) {
+            if ((rc = sort.run(c, pp, s)) != MPM_OK) return rc;
+        }
+        if ((rc = c->launch(KID_P2G, [&] {
+                 k_p2g<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, R(G), n, fl);
+             })) != MPM_OK)
+            return rc;
+        if ((rc = c->launch(KID_GRID1, [&] {
+                 k_grid1<DIM, R><<<gb, kBlock, 0, s>>>(geom, gp, sa, int(nc));
+             })) != MPM_OK)
+            return rc;
+        if ((rc = c->launch(KID_G2P1, [&] {
+                 k_g2p1_p2g2<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, sa, n, fl);
+             })) != MPM_OK)
+            return rc;
+        if ((rc = c->launch(KID_GRID2, [&] {
+                 k_grid2<DIM, R><<<nblocks(ni), kBlock, 0, s>>>(geom, gp, R(dt));
+             })) != MPM_OK)
+            return rc;
+        if ((rc = c->launch(KID_G2P2, [&] {
+                 k_g2p2<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, d_mats, sa, n, fl);
+             })) != MPM_OK)
+            return rc;
+        if (ar.mvl) {
+            if ((rc = c->launch(KID_MVL2, [&] {
+                     k_mvl2<DIM, R><<<pb, kBlock, 0, s>>>(pp, gp, n);
+                 })) != MPM_OK)
+                return rc;
+        }
+        c->steps += 1;
+        return MPM_OK;
+    }
+
+    int step(mpm_ctx* c, const mpm_args& a, double dt, double Ti) override {
+        const bool lin = c->cfg.basis == MPM_LINEAR;
+        if (dim == 2) return lin ? step_impl<2, 2>(c, a, dt, Ti) : step_impl<2, 3>(c, a, dt, Ti);
+        return lin ? step_impl<3, 2>(c, a, dt, Ti) : step_impl<3, 3>(c, a, dt, Ti);
+    }
+
+    template <int DIM, int S>
+    int count_impl(mpm_ctx* c, int64_t* out) {
+        unsigned char* mark = nullptr;
+        unsigned long long* cnt = nullptr;
+        hipError_t e = hipMallocAsync((void**)&mark, ni, c->stream);
+        if (e == hipSuccess) e = hipMallocAsync((void**)&cnt, sizeof(unsigned long long), c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(mark, 0, ni, c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream);
+        if (e == hipSuccess) {
+            k_mark_active<DIM, S, R><<<nblocks(np), kBlock, 0, c->stream>>>(geom, pp, mark, int(np));
+            k_count_bytes<<<1024, 256, 0, c->stream>>>(mark, ni, cnt);
+            e = hipGetLastError();
+        }
+        unsigned long long h = 0;
+        if (e == hipSuccess) e = hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+        if (mark) hipFreeAsync(mark, c->stream);
+        if (cnt) hipFreeAsync(cnt, c->stream);
+        hipError_t e2 = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = e2;
+        if (e != hipSuccess) return c->hip_fail(e, "mpm_count_active_nodes");
+        *out = (int64_t)h;
+        return MPM_OK;
+    }
+
+    int count_active(mpm_ctx* c, int64_t* out) override {
+        const bool lin = c->cfg.basis == MPM_LINEAR;
+        if (dim == 2) return lin ? count_impl<2, 2>(c, out) : count_impl<2, 3>(c, out);
+        return lin ? count_impl<3, 2>(c, out) : count_impl<3, 3>(c, out);
+    }
+
+    template <typename T>
+    int fetch(mpm_ctx* c, const T* src, double* host, const char* what) {
+        const int n = int(np);
+        k_unpermute<T><<<nblocks(np), kBlock, 0, c->stream>>>(src, pp.pid, d_stage, n);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(host, d_stage, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        return e == hipSuccess ? MPM_OK : c->hip_fail(e, what);
+    }
+
+    int download(mpm_ctx* c, double* xi, double* vs, double* sigma, double* sigma_m,
+                 double* omega, double* eps_q, double* eps_k, double* eps_v, double* eps_s,
+                 double* ms) override {
+        int rc;
+        for (int a = 0; a < dim; ++a) {
+            if (xi && (rc = fetch(c, pp.x[a], xi + a * np, "download xi")) != MPM_OK) return rc;
+            if (vs && (rc = fetch(c, pp.v[a], vs + a * np, "download vs")) != MPM_OK) return rc;
+        }
+        for (int k = 0; k < ns; ++k) {
+            if (sigma && (rc = fetch(c, pp.sig[k], sigma + k * np, "download sigma")) != MPM_OK)
+                return rc;
+            if (eps_s && (rc = fetch(c, pp.eps[k], eps_s + k * np, "download eps_s")) != MPM_OK)
+                return rc;
+        }
+        if (sigma_m) {
+            k_sigm<R><<<nblocks(np), kBlock, 0, c->stream>>>(pp.sig[0], pp.sig[1], pp.sig[2],
+                                                             pp.pid, d_stage, int(np));
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(sigma_m, d_stage, sizeof(double) * np, hipMemcpyDeviceToHost,
+                                   c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) return c->hip_fail(e, "download sigma_m");
+        }
+        if (omega && (rc = fetch(c, pp.vol, omega, "download omega")) != MPM_OK) return rc;
+        if (eps_q && (rc = fetch(c, pp.eq, eps_q, "download eps_q")) != MPM_OK) return rc;
+        if (eps_k && (rc = fetch(c, pp.ek, eps_k, "download eps_k")) != MPM_OK) return rc;
+        if (eps_v && (rc = fetch(c, pp.ev, eps_v, "download eps_v")) != MPM_OK) return rc;
+        if (ms && (rc = fetch(c, pp.ms, ms, "download ms")) != MPM_OK) return rc;
+        return MPM_OK;
+    }
+};
+
+int validate(const mpm_config* c, std::string& why) {
+    if (!c) { why = "NULL config"; return MPM_E_ARG; }
+    if (c->dim != 2 && c->dim != 3) { why = "dim must be 2 or 3"; return MPM_E_ARG; }
+    if (c->precision != MPM_FP64 && c->precision != MPM_FP32) { why = "precision must be MPM_FP64 or MPM_FP32"; return MPM_E_ARG; }
+    if (c->basis != MPM_UGIMP && c->basis != MPM_LINEAR) { why = "basis must be uGIMP or linear (gslinear has no kernels in the reference either)"; return MPM_E_UNSUPPORTED; }
+    if (c->np <= 0 || c->np > 0x7fffffffLL) { why = "np must be in 1..2^31-1"; return MPM_E_ARG; }
+    if (c->nmat <= 0) { why = "nmat must be positive"; return MPM_E_ARG; }
+    if (c->nnx < 3 || c->nny < 3 || (c->dim == 3 && c->nnz < 3)) { why = "grid needs at least 3 nodes per axis"; return MPM_E_ARG; }
+    double ni = double(c->nnx) * double(c->nny) * double(c->dim == 3 ? c->nnz : 1);
+    if (ni > 2.0e9) { why = "grid larger than 2e9 nodes"; return MPM_E_ARG; }
+    if (!(c->dx > 0 && c->dy > 0 && (c->dim == 2 || c->dz > 0))) { why = "grid spacing must be positive"; return MPM_E_ARG; }
+    if (c->basis == MPM_UGIMP) {
+        if (!(c->lpx > 0 && c->lpy > 0 && (c->dim == 2 || c->lpz > 0))) { why = "particle spacing must be positive"; return MPM_E_ARG; }
+        if (c->lpx > c->dx || c->lpy > c->dy || (c->dim == 3 && c->lpz > c->dz)) { why = "uGIMP needs particle spacing <= grid spacing"; return MPM_E_ARG; }
+    }
+    if (c->sort_interval < 0) { why = "sort_interval must be >= 0"; return MPM_E_ARG; }
+    return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_abi_version(void) { return MPM_ABI_VERSION; }
+
+int mpm_create(const mpm_config* cfg, mpm_ctx** out) {
+    if (!out) {
+        g_create_error = "mpm_create: NULL out";
+        return MPM_E_ARG;
+    }
+    *out = nullptr;
+    std::string why;
+    int rc = validate(cfg, why);
+    if (rc != MPM_OK) {
+        g_create_error = "mpm_create: " + why;
+        return rc;
+    }
+    mpm_ctx* c = new mpm_ctx();
+    c->cfg = *cfg;
+    if (c->cfg.dim == 2) {
+        c->cfg.nnz = 1;
+        c->cfg.z1 = 0.0;
+        c->cfg.dz = 1.0;
+        c->cfg.lpz = 1.0;
+    }
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        g_create_error = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+        delete c;
+        return MPM_E_HIP;
+    }
+    e = hipMalloc((void**)&c->d_flags, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->d_flags, 0, sizeof(int));
+    if (e != hipSuccess) {
+        g_create_error = std::string("hipMalloc flags: ") + hipGetErrorString(e);
+        hipStreamDestroy(c->stream);
+        delete c;
+        return MPM_E_HIP;
+    }
+    if (c->cfg.precision == MPM_FP64) {
+        auto* im = new Impl<double>();
+        c->impl = im;
+        rc = im->init(c);
+    } else {
+        auto* im = new Impl<float>();
+        c->impl = im;
+        rc = im->init(c);
+    }
+    if (rc != MPM_OK) {
+        g_create_error = "mpm_create: " + c->err;
+        mpm_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return MPM_OK;
+}
+
+int mpm_upload_particles(mpm_ctx* c, const double* xi, const double* vs, const double* sigma,
+                         const double* eps_s, const double* F, const double* omega,
+                         const double* omega0, const double* rho, const double* rho0,
+                         const double* eps_q, const double* eps_k, const int64_t* nid) {
+    if (!c) return MPM_E_ARG;
+    int rc = c->impl->upload_particles(c, xi, vs, sigma, eps_s, F, omega, omega0, rho, rho0,
+                                       eps_q, eps_k, nid);
+    if (rc == MPM_OK) c->have_particles = true;
+    return rc;
+}
+
+int mpm_upload_materials(mpm_ctx* c, const double* Ks, const double* Gs, const double* phi,
+                         const double* psi, const double* coh, const double* sigma_t,
+                         const double* Hp, const double* cr) {
+    if (!c) return MPM_E_ARG;
+    int rc = c->impl->upload_materials(c, Ks, Gs, phi, psi, coh, sigma_t, Hp, cr);
+    if (rc == MPM_OK) c->have_materials = true;
+    return rc;
+}
+
+int mpm_upload_bc(mpm_ctx* c, const int64_t* vx_idx, const double* vx_val, const int64_t* vy_idx,
+                  const double* vy_val, const int64_t* vz_idx, const double* vz_val) {
+    if (!c) return MPM_E_ARG;
+    int rc = c->impl->upload_bc(c, vx_idx, vx_val, vy_idx, vy_val, vz_idx, vz_val);
+    if (rc == MPM_OK) c->have_bc = true;
+    return rc;
+}
+
+int mpm_step(mpm_ctx* c, const mpm_args* a, double dt, double Ti) {
+    if (!c) return MPM_E_ARG;
+    if (!a) return c->fail(MPM_E_ARG, "mpm_step: NULL args");
+    if (!c->have_particles || !c->have_materials || !c->have_bc)
+        return c->fail(MPM_E_STATE, "mpm_step: upload particles, materials and bc first");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return c->fail(MPM_E_ARG, "mpm_step: dt must be positive and finite");
+    if (a->scheme != MPM_MUSL) return c->fail(MPM_E_UNSUPPORTED, "mpm_step: only the MUSL scheme is implemented");
+    if (a->va != MPM_VA_A && a->va != MPM_VA_V) return c->fail(MPM_E_ARG, "mpm_step: va must be :a or :v");
+    if (a->constitutive < 0 || a->constitutive > 2)
+        return c->fail(MPM_E_UNSUPPORTED, "mpm_step: constitutive must be linearelastic, druckerprager or mohrcoulomb");
+    return c->impl->step(c, *a, dt, Ti);
+}
+
+int mpm_run(mpm_ctx* c, const mpm_args* a, double dt, double Ti, int64_t nsteps, double* Ti_out) {
+    if (!c) return MPM_E_ARG;
+    if (nsteps < 0) return c->fail(MPM_E_ARG, "mpm_run: nsteps < 0");
+    for (int64_t k = 0; k < nsteps; ++k) {
+        int rc = mpm_step(c, a, dt, Ti);
+        if (rc != MPM_OK) return rc;
+        Ti += dt;
+    }
+    if (Ti_out) *Ti_out = Ti;
+    return MPM_OK;
+}
+
+int mpm_sync(mpm_ctx* c) {
+    if (!c) return MPM_E_ARG;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "mpm_sync");
+    int flags = 0;
+    e = hipMemcpy(&flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return c->hip_fail(e, "mpm_sync flags");
+    if (flags) {
+        hipMemset(c->d_flags, 0, sizeof(int));
+        std::string what;
+        if (flags & FLAG_RANGE) what += "a particle's basis stencil left the grid (pad the grid or enlarge it)";
+        if (flags & FLAG_NONFINITE) what += std::string(what.empty() ? "" : "; ") + "a particle position became non-finite";
+        return c->fail(MPM_E_RANGE, "mpm_sync: " + what + "; such particles are frozen");
+    }
+    return MPM_OK;
+}
+
+int mpm_download_particles(mpm_ctx* c, double* xi, double* vs, double* sigma, double* sigma_m,
+                           double* omega, double* eps_q, double* eps_k, double* eps_v,
+                           double* eps_s, double* ms) {
+    if (!c) return MPM_E_ARG;
+    if (!c->have_particles) return c->fail(MPM_E_STATE, "mpm_download_particles: nothing uploaded");
+    return c->impl->download(c, xi, vs, sigma, sigma_m, omega, eps_q, eps_k, eps_v, eps_s, ms);
+}
+
+void mpm_destroy(mpm_ctx* c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    delete c->impl;
+    for (auto& r : c->recs) {
+        hipEventDestroy(r.e0);
+        hipEventDestroy(r.e1);
+    }
+    for (auto e : c->pool) hipEventDestroy(e);
+    if (c->d_flags) hipFree(c->d_flags);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* mpm_last_error(const mpm_ctx* c) {
+    return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+int mpm_profile_enable(mpm_ctx* c, int on) {
+    if (!c) return MPM_E_ARG;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "mpm_profile_enable");
+    for (auto& r : c->recs) {
+        c->pool.push_back(r.e0);
+        c->pool.push_back(r.e1);
+    }
+    c->recs.clear();
+    c->prof_on = on != 0;
+    return MPM_OK;
+}
+
+int mpm_profile_read(mpm_ctx* c, double* total_ms, int64_t* launches, int n) {
+    if (!c || n < 0) return MPM_E_ARG;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "mpm_profile_read");
+    for (int k = 0; k < n; ++k) {
+        if (total_ms) total_ms[k] = 0.0;
+        if (launches) launches[k] = 0;
+    }
+    for (auto& r : c->recs) {
+        float ms = 0.f;
+        e = hipEventElapsedTime(&ms, r.e0, r.e1);
+        if (e != hipSuccess) return c->hip_fail(e, "hipEventElapsedTime");
+        if (r.kid < n) {
+            if (total_ms) total_ms[r.kid] += ms;
+            if (launches) launches[r.kid] += 1;
+        }
+    }
+    return MPM_OK;
+}
+
+const char* mpm_profile_kernel_name(int k) {
+    return (k >= 0 && k < kNumKernels) ? kKernelNames[k] : "";
+}
+
+int mpm_profile_kernel_count(void) { return kNumKernels; }
+
+int mpm_count_active_nodes(mpm_ctx* c, int64_t* count) {
+    if (!c || !count) return MPM_E_ARG;
+    if (!c->have_particles) return c->fail(MPM_E_STATE, "mpm_count_active_nodes: nothing uploaded");
+    return c->impl->count_active(c, count);
+}
+
+int mpm_tpeak(int64_t nbytes, int reps, double* gbps) {
+    if (nbytes < 1024 || reps < 1 || !gbps) return MPM_E_ARG;
+    long long n4 = nbytes / 16;
+    float4 *a = nullptr, *b = nullptr, *cc = nullptr;
+    hipError_t e = hipMalloc((void**)&a, n4 * 16);
+    if (e == hipSuccess) e = hipMalloc((void**)&b, n4 * 16);
+    if (e == hipSuccess) e = hipMalloc((void**)&cc, n4 * 16);
+    if (e == hipSuccess) e = hipMemset(b, 0, n4 * 16);
+    if (e == hipSuccess) e = hipMemset(cc, 0, n4 * 16);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    double best = 0.0;
+    unsigned blocks = 256 * 32;
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        hipEventRecord(e0, 0);
+        k_triad<<<blocks, 256, 0, 0>>>(b, cc, a, n4);
+        hipEventRecord(e1, 0);
+        e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        double g = 3.0 * double(n4) * 16.0 / (double(ms) * 1e-3) / 1e9;
+        if (g > best) best = g;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    if (cc) hipFree(cc);
+    if (e != hipSuccess) return MPM_E_HIP;
+    *gbps = best;
+    return MPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,737 @@
+// mpm_kernels.hpp -- gfx950 kernels of the explicit single-phase MUSL step.
+//
+// One step is five launches (six with MVL):
+//   k_p2g        P2G #1: mass, momentum, internal + external force
+//                (P2G_OS!, src/solvers/utils_OS.jl:424-496, with the basis of
+//                resetmpstatus_OS!, utils_OS.jl:178-415, evaluated on the fly)
+//   k_grid1      grid solve #1 (solvegrid_a_OS! / solvegrid_OS!,
+//                utils_OS.jl:507-687); also clears f for the next step
+//   k_g2p1_p2g2  MUSL re-map: gather vsT, as, vs, move the particle, scatter
+//                the new momentum with the SAME step-start basis
+//                (doublemapping1(_a)_OS! + doublemapping2_OS!, utils_OS.jl:698-959)
+//   k_grid2      grid solve #2 (doublemapping3_OS!, utils_OS.jl:968-1016); also
+//                clears m and p for the next step
+//   k_g2p2       G2P with the step-start basis, strain / J update, constitutive
+//                update and position commit (G2P_OS!, utils_OS.jl:1025-1153;
+//                liE!, dpP!, mcP!)
+//   k_mvl2       MVL second half (vollock2_OS!, utils_OS.jl:1202-1256); the
+//                first half (vollock1_OS!) is fused into k_g2p2
+//
+// The basis is never stored: each particle pass recomputes the S per-axis
+// weights (S = 3 for uGIMP, 2 for linear) from the particle position, so a
+// pass streams only the particle state it needs (SoA, coalesced).  Particles
+// whose stencil would leave the grid, or whose position is not finite, are
+// frozen and reported through a device flag word instead of being allowed to
+// read or write outside the grid arrays.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpm {
+
+constexpr double kEpsF64 = 2.220446049250313e-16;  // eps(Float64): massless-node cut
+
+enum : int { FLAG_RANGE = 1, FLAG_NONFINITE = 2 };
+
+// ---------------------------------------------------------------------------
+// Parameters passed by value
+// ---------------------------------------------------------------------------
+struct GridGeom {
+    int nn[3];        // nodes per axis (nn[2] = 1 in 2D)
+    int nc[3];        // cells per axis
+    long long ni;     // nn[0]*nn[1]*nn[2]
+    double org[3];    // x1, y1, z1
+    double h[3];      // dx, dy, dz
+    double hinv[3];   // 1/h
+    double lp[3];     // particle domain size
+    // uGIMP constants per axis (mpbasisfunc.jl:56-120)
+    double c4[3];     // 1/(4 h lp)
+    double c1[3];     // 1/(h lp)
+    double half[3];   // 0.5 lp
+};
+
+template <typename R>
+struct PartPtrs {
+    double* x[3];     // positions (FP64 in every precision)
+    R* v[3];          // velocity
+    R* sig[6];        // Cauchy stress, Voigt [xx,yy,zz,xy,yz,xz] (2D: [xx,yy,zz,xy])
+    R* ms;            // mass (constant: Ω·ρs at upload)
+    R* vol;           // Ω = J Ω0
+    R* vol0;          // Ω0
+    double* J;        // det F (FP64 in every precision)
+    R* eq;            // ϵq
+    R* ek;            // ϵk
+    R* ev;            // ϵv
+    R* eps[6];        // ϵijs (engineering shear)
+    R* dxp[3];        // MUSL scratch: position increment of the current step
+    int* nid;         // 0-based material id
+    int* pid;         // original particle index (for downloads after sorting)
+    int* p2c;         // step-start cell id (MVL only)
+};
+
+template <typename R>
+struct GridPtrs {
+    R* m;
+    R* p[3];
+    R* f[3];
+    R* vs[3];
+    R* vsT[3];
+    R* as[3];
+    R* du[3];
+    unsigned char* bcf;   // bit d set: axis d velocity fixed
+    R* bcv[3];            // fixed values
+    R* cell_sm;           // MVL: Σ Ω σm per cell
+    R* cell_vol;          // MVL: Σ Ω per cell
+};
+
+// Material constants, precomputed on the host in FP64 with the reference's
+// expressions (druckerprager.jl:178-183, linearelastic.jl:48-49) and stored
+// in the working precision.
+template <typename R>
+struct Material {
+    R Ks, Gs, Dt, Dd;
+    R kphi, qphi, qpsi, sigt, ab, tb;   // Drucker-Prager
+    R eq_shear;                          // sqrt(0.333333 + 0.222222 qψ²)
+    R sinphi, cosphi, tanphi, c, Hp, cr; // Mohr-Coulomb (2D)
+};
+
+template <typename R>
+struct StepArgs {
+    R dt, G, flip, pic, zeta;
+    int va;            // 0 = :a, 1 = :v
+    int plastic;       // Ti >= Te
+    int constitutive;  // 0 LE, 1 DP, 2 MC
+    int mvl;
+};
+
+// ---------------------------------------------------------------------------
+// Basis functions
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ R sgn(R x) {
+    return x > R(0) ? R(1) : (x < R(0) ? R(-1) : x);
+}
+
+// uGIMP weight and derivative (uGIMPbasis, mpbasisfunc.jl:40-53).  The FP64
+// form keeps the reference's expression so parity runs agree to rounding;
+// the FP32 form uses the reciprocal constants of uGIMPbasisx
+// (mpbasisfunc.jl:56-76).
+template <typename R>
+__device__ __forceinline__ void ugimp(R d, R h, R lp, R c4, R c1, R half, R& N, R& dN) {
+    R a = d < R(0) ? -d : d;
+    N = R(0);
+    dN = R(0);
+    if constexpr (sizeof(R) == 8) {
+        if (a < R(0.5) * lp) {
+            N = R(1) - ((R(4) * d * d + lp * lp) / (R(4) * h * lp));
+            dN = -((R(8) * d) / (R(4) * h * lp));
+        } else if (R(0.5) * lp <= a && a < (h - R(0.5) * lp)) {
+            N = R(1) - (a / h);
+            dN = sgn(d) * (R(-1) / h);
+        } else if ((h - R(0.5) * lp) <= a && a < (h + R(0.5) * lp)) {
+            N = ((h + R(0.5) * lp - a) * (h + R(0.5) * lp - a)) / (R(2) * h * lp);
+            dN = -sgn(d) * ((h + R(0.5) * lp - a) / (h * lp));
+        }
+    } else {
+        if (a < half) {
+            N = R(1) - (R(4) * d * d + lp * lp) * c4;
+            dN = R(-8) * d * c4;
+        } else if (a < h - half) {
+            N = R(1) - a / h;
+            dN = -sgn(d) / h;
+        } else if (a < h + half) {
+            R t = h + half - a;
+            N = t * t * R(0.5) * c1;
+            dN = -sgn(d) * t * c1;
+        }
+    }
+}
+
+// Per-axis stencil: S nodes starting at base[a].
+template <int DIM, int S, typename R>
+struct Stencil {
+    int base[3];
+    R N[3][S];
+    R dN[3][S];
+};
+
+// Builds the stencil of a particle at x.  uGIMP (S = 3): the nodes with
+// |x - x_n| < h + lp/2 lie in {floor(u - 1/2) + k, k = 0..2}, u = (x - x1)/h,
+// for any lp <= h; the weight function itself returns zero outside the
+// support, so the window may hold one zero-weight node.  Linear (S = 2): the
+// two corners of the reference's cell (fld on x and z, cld on y,
+// utils_OS.jl:124-127).  Returns false when the window leaves the grid or x is
+// not finite.
+template <int DIM, int S, typename R>
+__device__ __forceinline__ bool make_stencil(const GridGeom& g, const double* x,
+                                             Stencil<DIM, S, R>& st, int& flag) {
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+        double u = (x[a] - g.org[a]) * g.hinv[a];
+        if (!isfinite(u)) {
+            flag |= FLAG_NONFINITE;
+            return false;
+        }
+        int b;
+        if constexpr (S == 3) {
+            double fb = floor(u - 0.5);
+            if (fb < 0.0 || fb > double(g.nn[a] - 3)) { ok = false; b = 0; }
+            else b = int(fb);
+        } else {
+            double fb = (a == 1) ? ceil(u) - 1.0 : floor(u);
+            if (fb < 0.0 || fb > double(g.nn[a] - 2)) { ok = false; b = 0; }
+            else b = int(fb);
+        }
+        st.base[a] = b;
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            double xn = g.org[a] + double(b + k) * g.h[a];
+            R d = R(x[a] - xn);
+            if constexpr (S == 3) {
+                ugimp<R>(d, R(g.h[a]), R(g.lp[a]), R(g.c4[a]), R(g.c1[a]), R(g.half[a]),
+                         st.N[a][k], st.dN[a][k]);
+            } else {
+                R hd = R(1) / R(g.h[a]);
+                R ad = d < R(0) ? -d : d;
+                st.N[a][k] = R(1) - ad * hd;
+                st.dN[a][k] = -sgn(d) * hd;
+            }
+        }
+    }
+    if (!ok) flag |= FLAG_RANGE;
+    return ok;
+}
+
+__device__ __forceinline__ long long node_id(const GridGeom& g, int ix, int iy, int iz) {
+    return (long long)ix + (long long)g.nn[0] * ((long long)iy + (long long)g.nn[1] * iz);
+}
+
+template <typename R>
+__device__ __forceinline__ void atomic_add(R* a, R v) {
+    atomicAdd(a, v);  // global_atomic_add_f32 / _f64 (built with -munsafe-fp-atomics)
+}
+
+__device__ __forceinline__ void raise_flag(int* flags, int f) {
+    if (f) atomicOr(flags, f);
+}
+
+// Iterate the DIM-dimensional stencil; body(n, N, dNx, dNy, dNz).  The weight
+// products follow the reference's factor order (utils_OS.jl:351-354).
+template <int DIM, int S, typename R, typename F>
+__device__ __forceinline__ void for_stencil(const GridGeom& g, const Stencil<DIM, S, R>& st,
+                                            F&& body) {
+    if constexpr (DIM == 2) {
+#pragma unroll
+        for (int a = 0; a < S; ++a)
+#pragma unroll
+            for (int b = 0; b < S; ++b) {
+                R Nx = st.N[0][a], Ny = st.N[1][b];
+                R N = Nx * Ny;
+                R dNx = st.dN[0][a] * Ny;
+                R dNy = st.dN[1][b] * Nx;
+                long long n = node_id(g, st.base[0] + a, st.base[1] + b, 0);
+                body(n, N, dNx, dNy, R(0));
+            }
+    } else {
+#pragma unroll
+        for (int c = 0; c < S; ++c)
+#pragma unroll
+            for (int b = 0; b < S; ++b)
+#pragma unroll
+                for (int a = 0; a < S; ++a) {
+                    R Nx = st.N[0][a], Ny = st.N[1][b], Nz = st.N[2][c];
+                    R N = Nx * Ny * Nz;
+                    R dNx = st.dN[0][a] * Ny * Nz;
+                    R dNy = st.dN[1][b] * Nx * Nz;
+                    R dNz = st.dN[2][c] * Nx * Ny;
+                    long long n = node_id(g, st.base[0] + a, st.base[1] + b, st.base[2] + c);
+                    body(n, N, dNx, dNy, dNz);
+                }
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void load_x(const double* const* xs, int i, double* x) {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) x[a] = xs[a][i];
+}
+
+// ---------------------------------------------------------------------------
+// P2G #1 (P2G_OS!, utils_OS.jl:424-496)
+// ---------------------------------------------------------------------------
+template <int DIM, int S, typename R>
+__global__ __launch_bounds__(256) void k_p2g(GridGeom g, PartPtrs<R> pp, GridPtrs<R> gp,
+                                             R G, int np, int* flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    double x[3];
+    load_x<DIM>(pp.x, i, x);
+    Stencil<DIM, S, R> st;
+    int flag = 0;
+    if (!make_stencil<DIM, S, R>(g, x, st, flag)) {
+        raise_flag(flags, flag);
+        return;
+    }
+    const R m = pp.ms[i], vol = pp.vol[i];
+    R pv[3];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) pv[a] = m * pp.v[a][i];
+    R s0 = pp.sig[0][i], s1 = pp.sig[1][i], s2 = R(0), s3 = pp.sig[3][i], s4 = R(0), s5 = R(0);
+    if constexpr (DIM == 3) {
+        s2 = pp.sig[2][i];
+        s4 = pp.sig[4][i];
+        s5 = pp.sig[5][i];
+    }
+    for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R dNx, R dNy, R dNz) {
+        if (N != R(0)) {
+            R NiM = m * N;
+            atomic_add(&gp.m[n], NiM);
+            atomic_add(&gp.p[0][n], N * pv[0]);
+            atomic_add(&gp.p[1][n], N * pv[1]);
+            if constexpr (DIM == 2) {
+                atomic_add(&gp.f[0][n], -vol * (dNx * s0 + dNy * s3));
+                atomic_add(&gp.f[1][n], -vol * (dNy * s1 + dNx * s3) + NiM * G);
+            } else {
+                atomic_add(&gp.p[2][n], N * pv[2]);
+                atomic_add(&gp.f[0][n], -vol * (dNx * s0 + dNy * s3 + dNz * s5));
+                atomic_add(&gp.f[1][n], -vol * (dNy * s1 + dNx * s3 + dNz * s4));
+                atomic_add(&gp.f[2][n], -vol * (dNz * s2 + dNx * s5 + dNy * s4) + NiM * G);
+            }
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Grid solve #1 (solvegrid_a_OS! utils_OS.jl:547-640; solvegrid_OS! 507-535,650-687)
+// Leaves p = 0 (reference) and f = 0 (next step's P2G accumulates into it);
+// clears the MVL cell sums (resetgridstatus_OS!, utils_OS.jl:82-99).
+// ---------------------------------------------------------------------------
+template <int DIM, typename R>
+__global__ __launch_bounds__(256) void k_grid1(GridGeom g, GridPtrs<R> gp, StepArgs<R> sa,
+                                               int ncells) {
+    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < ncells && sa.mvl) {
+        gp.cell_sm[n] = R(0);
+        gp.cell_vol[n] = R(0);
+    }
+    if (n >= g.ni) return;
+    const R m = gp.m[n];
+    const R msd = m < R(kEpsF64) ? R(0) : R(1) / m;
+    const unsigned bc = gp.bcf[n];
+    R P[3], Fc[3], vs[3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) P[d] = gp.p[d][n];
+    R ss = R(0);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        R fd = gp.f[d][n];
+        Fc[d] = fd;
+        ss += fd * fd;
+    }
+    const R damp = -sa.zeta * sqrt(ss);
+    if (sa.va == 0) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+            if (bc & (1u << d)) P[d] = gp.bcv[d][n];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            vs[d] = P[d] * msd;
+            R Fs = Fc[d] + damp * sgn(vs[d]);
+            R vT = vs[d] + Fs * sa.dt * msd;
+            R ac = Fs * msd;
+            if (bc & (1u << d)) {
+                vT = gp.bcv[d][n];
+                ac = gp.bcv[d][n];
+            }
+            gp.vs[d][n] = vs[d];
+            gp.vsT[d][n] = vT;
+            gp.as[d][n] = ac;
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            vs[d] = P[d] * msd;
+            R Fs = Fc[d] + damp * sgn(vs[d]);
+            R vT = (P[d] + Fs * sa.dt) * msd;
+            if (bc & (1u << d)) vT = gp.bcv[d][n];
+            gp.vs[d][n] = vs[d];
+            gp.vsT[d][n] = vT;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        gp.p[d][n] = R(0);
+        gp.f[d][n] = R(0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MUSL re-map (doublemapping1(_a)_OS! + doublemapping2_OS!, utils_OS.jl:698-959)
+// ---------------------------------------------------------------------------
+template <int DIM, int S, typename R>
+__global__ __launch_bounds__(256) void k_g2p1_p2g2(GridGeom g, PartPtrs<R> pp, GridPtrs<R> gp,
+                                                   StepArgs<R> sa, int np, int* flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    double x[3];
+    load_x<DIM>(pp.x, i, x);
+    Stencil<DIM, S, R> st;
+    int flag = 0;
+    if (!make_stencil<DIM, S, R>(g, x, st, flag)) {
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) pp.dxp[a][i] = R(0);
+        return;  // already reported by k_p2g
+    }
+    R A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, C[3] = {0, 0, 0};
+    for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
+        if (N != R(0)) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                R vT = gp.vsT[d][n];
+                if (sa.va == 0) {
+                    A[d] += N * vT;
+                    B[d] += N * gp.as[d][n];
+                    C[d] += N * gp.vs[d][n];
+                } else {
+                    A[d] += N * vT;
+                    B[d] += N * (vT - gp.vs[d][n]);
+                    C[d] += N * vT;
+                }
+            }
+        }
+    });
+    const R m = pp.ms[i];
+    R pv[3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        R v = pp.v[d][i];
+        R vn = (sa.va == 0) ? sa.flip * v + sa.pic * C[d] + B[d] * sa.dt
+                            : sa.flip * (v + B[d]) + sa.pic * C[d];
+        pp.v[d][i] = vn;
+        pp.dxp[d][i] = A[d] * sa.dt;
+        pv[d] = m * vn;
+    }
+    for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
+        if (N != R(0)) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) atomic_add(&gp.p[d][n], pv[d] * N);
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Grid solve #2 (doublemapping3_OS!, utils_OS.jl:968-1016); clears m and p.
+// ---------------------------------------------------------------------------
+template <int DIM, typename R>
+__global__ __launch_bounds__(256) void k_grid2(GridGeom g, GridPtrs<R> gp, R dt) {
+    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.ni) return;
+    const R m = gp.m[n];
+    const R msd = m < R(kEpsF64) ? R(0) : R(1) / m;
+    const unsigned bc = gp.bcf[n];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        R v = gp.p[d][n] * msd;
+        if (bc & (1u << d)) v = gp.bcv[d][n];
+        gp.du[d][n] = v * dt;
+        gp.p[d][n] = R(0);
+    }
+    gp.m[n] = R(0);
+}
+
+// ---------------------------------------------------------------------------
+// Constitutive updates on registers
+// ---------------------------------------------------------------------------
+template <int NS, typename R>
+__device__ __forceinline__ R mean3(const R* s) {
+    return (s[0] + s[1] + s[2]) * R(0.333333);
+}
+
+// liE! (linearelastic.jl:22-69 2D, 78-140 3D).  F holds ΔF row-major
+// (2D: [dF1 dF2; dF3 dF4], 3D: 9 entries), e the strain increment.
+template <int DIM, typename R>
+__device__ __forceinline__ void liE(R* s, const R* F, const R* e, const Material<R>& M) {
+    if constexpr (DIM == 2) {
+        R wxy = R(0.5) * (F[1] - F[2]);
+        R s1 = s[0], s2 = s[1], s4 = s[3];
+        s[0] += wxy * s4 * R(2);
+        s[1] += -wxy * s4 * R(2);
+        s[3] += wxy * (s2 - s1);
+        s[0] += M.Dt * e[0] + M.Dd * e[1] + M.Dd * e[2];
+        s[1] += M.Dd * e[0] + M.Dt * e[1] + M.Dd * e[2];
+        s[2] += M.Dd * e[0] + M.Dd * e[1] + M.Dt * e[2];
+        s[3] += M.Gs * e[3];
+    } else {
+        R wxy = R(0.5) * (F[3] - F[1]);
+        R wyz = R(0.5) * (F[7] - F[5]);
+        R wxz = R(0.5) * (F[6] - F[2]);
+        R s1 = s[0], s2 = s[1], s3 = s[2], s4 = s[3], s5 = s[4], s6 = s[5];
+        s[0] += R(2) * (s4 * wxy + s6 * wxz);
+        s[1] += -R(2) * (s4 * wxy - s5 * wyz);
+        s[2] += -R(2) * (s6 * wxz + s5 * wyz);
+        s[3] += wxy * (s2 - s1) + wxz * s5 + wyz * s6;
+        s[4] += wyz * (s3 - s2) - wxz * s4 - wxy * s6;
+        s[5] += wxz * (s3 - s1) + wxy * s5 - wyz * s4;
+        s[0] += M.Dt * e[0] + M.Dd * e[1] + M.Dd * e[2];
+        s[1] += M.Dd * e[0] + M.Dt * e[1] + M.Dd * e[2];
+        s[2] += M.Dd * e[0] + M.Dd * e[1] + M.Dt * e[2];
+        s[3] += M.Gs * e[3];
+        s[4] += M.Gs * e[4];
+        s[5] += M.Gs * e[5];
+    }
+}
+
+// dpP! (druckerprager.jl:22-80 2D, 89-155 3D).
+template <int DIM, typename R>
+__device__ __forceinline__ void dpP(R* s, R& eq, R& ek, const Material<R>& M) {
+    constexpr int NS = DIM == 2 ? 4 : 6;
+    R sm = mean3<NS>(s);
+    R sij[NS];
+    sij[0] = s[0] - sm;
+    sij[1] = s[1] - sm;
+    sij[2] = s[2] - sm;
+#pragma unroll
+    for (int k = 3; k < NS; ++k) sij[k] = s[k];
+    R q = R(0.5) * (sij[0] * sij[0] + sij[1] * sij[1] + sij[2] * sij[2]);
+#pragma unroll
+    for (int k = 3; k < NS; ++k) q += sij[k] * sij[k];
+    const R tau = sqrt(q);
+    const R st = M.sigt;
+    const R fs = tau + M.qphi * sm - M.kphi;
+    const R ft = sm - st;
+    const R BF = (tau - M.tb) - (M.ab * (sm - st));
+    if (((sm < st) && (fs > R(0))) || ((sm >= st) && (BF > R(0)))) {
+        R dls = fs / (M.Gs + M.Ks * M.qphi * M.qpsi);
+        R t1 = sm - M.Ks * M.qpsi * dls;
+        R t2 = (M.kphi - M.qphi * t1) / tau;
+        s[0] = sij[0] * t2 + t1;
+        s[1] = sij[1] * t2 + t1;
+        s[2] = sij[2] * t2 + t1;
+#pragma unroll
+        for (int k = 3; k < NS; ++k) s[k] = sij[k] * t2;
+        eq += dls * M.eq_shear;
+        ek += dls * M.qpsi;
+    }
+    if ((sm >= st) && (BF <= R(0))) {
+        R dlt = ft / M.Ks;
+        s[0] = sij[0] + st;
+        s[1] = sij[1] + st;
+        s[2] = sij[2] + st;
+        eq += dlt * R(0.333333) * R(1.414214);
+        ek += dlt;
+    }
+}
+
+// mcP! 2D plane strain (mohrcoulomb.jl:22-86).  The reference's 3D mcP! is a
+// no-op (its body is commented out, mohrcoulomb.jl:152-455).
+template <typename R>
+__device__ __forceinline__ void mcP2(R* s, R& eq, R& ek, const Material<R>& M) {
+    R c = M.c + M.Hp * eq;
+    c = (c > M.cr || c != c) ? c : M.cr;  // max(c + Hp ϵq, cr)
+    const R ds = s[0] - s[1];
+    const R tau = sqrt(R(0.25) * (ds * ds) + s[3] * s[3]);
+    const R sg = R(0.5) * (s[0] + s[1]);
+    const R f = tau + sg * M.sinphi - c * M.cosphi;
+    R sn1 = s[0], sn2 = s[1], sn3 = s[2], sn4 = s[3];
+    R bb = c * M.cosphi - sg * M.sinphi;
+    const R beta = (bb < R(0) ? -bb : bb) / tau;
+    const R dsigA = R(0.5) * beta * ds;
+    const R dsigB = c / M.tanphi;
+    if ((sg <= dsigB) && (f > R(0))) {
+        sn1 = sg + dsigA;
+        sn2 = sg - dsigA;
+        sn4 = beta * s[3];
+    }
+    if ((sg > dsigB) && (f > R(0))) {
+        sn1 = dsigB;
+        sn2 = dsigB;
+        sn4 = R(0);
+    }
+    const R d1 = sn1 - s[0], d2 = sn2 - s[1], d3 = sn3 - s[2], d4 = sn4 - s[3];
+    s[0] = sn1;
+    s[1] = sn2;
+    s[2] = sn3;
+    s[3] = sn4;
+    const R Dt = M.Dt, Dd = M.Dd;
+    const R base = R(1) / ((Dd - Dt) * (R(2) * Dd + Dt));
+    const R exx = -(Dd * d1 + Dt * d1 - Dd * d2 - Dd * d3) * base;
+    const R eyy = -(-Dd * d1 + Dd * d2 + Dt * d2 - Dd * d3) * base;
+    const R ezz = -(-Dd * d1 - Dd * d2 + Dd * d3 + Dt * d3) * base;
+    const R exy = d4 / M.Gs;
+    ek = exx + eyy + ezz;
+    eq += sqrt(R(0.666667) * (exx * exx + eyy * eyy + ezz * ezz + exy * exy * R(2)));
+}
+
+// Cell id of the reference's p2c (fld on x and z, cld on y), 0-based, in this
+// library's x-fastest cell order.
+template <int DIM>
+__device__ __forceinline__ int cell_of(const GridGeom& g, const double* x) {
+    int cx = int(floor((x[0] - g.org[0]) * g.hinv[0]));
+    int cy = int(ceil((x[1] - g.org[1]) * g.hinv[1])) - 1;
+    int cz = DIM == 3 ? int(floor((x[2] - g.org[2]) * g.hinv[2])) : 0;
+    return cx + g.nc[0] * (cy + g.nc[1] * cz);
+}
+
+// ---------------------------------------------------------------------------
+// G2P #2 + constitutive + commit (G2P_OS!, utils_OS.jl:1025-1153; liE!; dpP!;
+// mcP!; vollock1_OS!)
+// ---------------------------------------------------------------------------
+template <int DIM, int S, typename R>
+__global__ __launch_bounds__(256) void k_g2p2(GridGeom g, PartPtrs<R> pp, GridPtrs<R> gp,
+                                              const Material<R>* __restrict__ mats,
+                                              StepArgs<R> sa, int np, int* flags) {
+    constexpr int NS = DIM == 2 ? 4 : 6;
+    constexpr int NF = DIM * DIM;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    double x[3];
+    load_x<DIM>(pp.x, i, x);
+    Stencil<DIM, S, R> st;
+    int flag = 0;
+    if (!make_stencil<DIM, S, R>(g, x, st, flag)) {
+        if (sa.mvl) pp.p2c[i] = -1;  // frozen particle: no cell
+        return;
+    }
+    R F[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) F[k] = R(0);
+    for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R dNx, R dNy, R dNz) {
+        if (N != R(0)) {
+            if constexpr (DIM == 2) {
+                R u0 = gp.du[0][n], u1 = gp.du[1][n];
+                F[0] += u0 * dNx;
+                F[1] += u0 * dNy;
+                F[2] += u1 * dNx;
+                F[3] += u1 * dNy;
+            } else {
+                R u0 = gp.du[0][n], u1 = gp.du[1][n], u2 = gp.du[2][n];
+                F[0] += u0 * dNx; F[1] += u0 * dNy; F[2] += u0 * dNz;
+                F[3] += u1 * dNx; F[4] += u1 * dNy; F[5] += u1 * dNz;
+                F[6] += u2 * dNx; F[7] += u2 * dNy; F[8] += u2 * dNz;
+            }
+        }
+    });
+    const R rdt = R(1) / sa.dt;
+    R e[NS];
+    double detI;
+    if constexpr (DIM == 2) {
+        R exx = F[0] * rdt, eyy = F[3] * rdt, exy = R(0.5) * (F[1] + F[2]) * rdt;
+        pp.ev[i] = sqrt(exx * exx + eyy * eyy + R(2) * exy * exy);
+        e[0] = F[0];
+        e[1] = F[3];
+        e[2] = R(0);
+        e[3] = F[1] + F[2];
+        pp.eps[0][i] += e[0];
+        pp.eps[1][i] += e[1];
+        pp.eps[3][i] += e[3];
+        double a = 1.0 + double(F[0]), b = double(F[1]), c = double(F[2]), d = 1.0 + double(F[3]);
+        detI = a * d - b * c;
+    } else {
+        R exx = F[0] * rdt, eyy = F[4] * rdt, ezz = F[8] * rdt;
+        R exy = R(0.5) * (F[1] + F[3]) * rdt;
+        R eyz = R(0.5) * (F[5] + F[7]) * rdt;
+        R exz = R(0.5) * (F[2] + F[6]) * rdt;
+        pp.ev[i] = sqrt(exx * exx + eyy * eyy + ezz * ezz + R(2) * (exy * exy + eyz * eyz + exz * exz));
+        e[0] = F[0];
+        e[1] = F[4];
+        e[2] = F[8];
+        e[3] = F[1] + F[3];
+        e[4] = F[5] + F[7];
+        e[5] = F[2] + F[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) pp.eps[k][i] += e[k];
+        double A[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) A[k] = double(F[k]);
+        A[0] += 1.0;
+        A[4] += 1.0;
+        A[8] += 1.0;
+        detI = A[0] * A[4] * A[8] + A[1] * A[5] * A[6] + A[2] * A[3] * A[7] -
+               A[6] * A[4] * A[2] - A[7] * A[5] * A[0] - A[8] * A[3] * A[1];
+    }
+    const double J = pp.J[i] * detI;
+    pp.J[i] = J;
+    const R vol = R(J * double(pp.vol0[i]));
+    pp.vol[i] = vol;
+
+    const Material<R> M = mats[pp.nid[i]];
+    R s[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s[k] = pp.sig[k][i];
+    liE<DIM, R>(s, F, e, M);
+    if (sa.plastic && sa.constitutive != 0) {
+        R eq = pp.eq[i], ek = pp.ek[i];
+        if (sa.constitutive == 1) {
+            dpP<DIM, R>(s, eq, ek, M);
+        } else if constexpr (DIM == 2) {
+            mcP2<R>(s, eq, ek, M);
+        }
+        pp.eq[i] = eq;
+        pp.ek[i] = ek;
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) pp.sig[k][i] = s[k];
+    if (sa.mvl) {
+        int c = cell_of<DIM>(g, x);
+        pp.p2c[i] = c;
+        R sm = mean3<NS>(s);
+        atomic_add(&gp.cell_sm[c], vol * sm);
+        atomic_add(&gp.cell_vol[c], vol);
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) pp.x[d][i] = x[d] + double(pp.dxp[d][i]);
+    raise_flag(flags, flag);
+}
+
+// vollock2_OS! (utils_OS.jl:1202-1256)
+template <int DIM, typename R>
+__global__ __launch_bounds__(256) void k_mvl2(PartPtrs<R> pp, GridPtrs<R> gp, int np) {
+    constexpr int NS = DIM == 2 ? 4 : 6;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    R s[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s[k] = pp.sig[k][i];
+    R sm0 = mean3<NS>(s);
+    int c = pp.p2c[i];
+    if (c < 0) return;
+    R smc = gp.cell_sm[c] / gp.cell_vol[c];
+    pp.sig[0][i] = (s[0] - sm0) + smc;
+    pp.sig[1][i] = (s[1] - sm0) + smc;
+    pp.sig[2][i] = (s[2] - sm0) + smc;
+}
+
+// Marks every node with a non-zero weight from some particle (diagnostic).
+template <int DIM, int S, typename R>
+__global__ __launch_bounds__(256) void k_mark_active(GridGeom g, PartPtrs<R> pp,
+                                                     unsigned char* __restrict__ mark, int np) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    double x[3];
+    load_x<DIM>(pp.x, i, x);
+    Stencil<DIM, S, R> st;
+    int flag = 0;
+    if (!make_stencil<DIM, S, R>(g, x, st, flag)) return;
+    for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
+        if (N != R(0)) mark[n] = 1;
+    });
+}
+
+__global__ __launch_bounds__(256) void k_count_bytes(const unsigned char* __restrict__ mark,
+                                                     long long n, unsigned long long* out) {
+    __shared__ unsigned long long part[256];
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    unsigned long long c = 0;
+    for (; i < n; i += stride) c += mark[i];
+    part[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, part[0]);
+}
+
+}  // namespace mpm

@@ -1,0 +1,298 @@
+"""Solver entry and the HIP device plugin, mirroring MaterialPointSolver.jl.
+
+* ``materialpointsolver``  -- materialpointsolver! (src/MaterialPointSolver.jl:61-75)
+* ``submit_work``          -- submit_work! (src/solver.jl:37-131)
+* ``procedure``            -- procedure!(..., Val(:OS), Val(:MUSL), Val(:a|:v))
+                              (src/solvers/OS_MUSL.jl:12-102), one libmpm_hip step
+* ``host2device`` / ``device2host`` / ``clean_device`` / ``getBackend`` / ``Tpeak``
+                           -- the device plugin of ext/AMDExt/devicehelpfunc_amd.jl,
+                              for the new device symbol ``"HIP"``
+
+Julia dispatches these on ``Val(args.device)``; here the device string is an
+argument.  Only ``"HIP"`` has an implementation: every other device raises.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _capi as capi
+
+
+class HIPModel:
+    """Owner of one libmpm_hip context (the device copy of grid, mp, attr, bc)."""
+
+    def __init__(self, grid, mp, attr, bc, sort_interval=0):
+        lib = capi.load()
+        self.lib = lib
+        self.dim = grid.dim
+        self.np = int(mp.np)
+        if grid.dim != mp.dim or grid.dim != bc.dim:
+            raise ValueError("grid, mp and bc must have the same dimension")
+        if grid.ϵ != mp.ϵ:
+            raise ValueError("grid and mp must use the same ϵ")
+        cfg = capi.MPMConfig()
+        cfg.dim = grid.dim
+        cfg.precision = capi.MPM_FP32 if mp.ϵ == "FP32" else capi.MPM_FP64
+        lin = grid.NIC in (4, 8)
+        if (mp.NIC in (4, 8)) != lin:
+            raise ValueError("grid.NIC and mp.NIC disagree on the basis")
+        cfg.basis = capi.MPM_LINEAR if lin else capi.MPM_UGIMP
+        cfg.sort_interval = int(sort_interval or 0)
+        cfg.np = self.np
+        cfg.nmat = int(attr.nmat)
+        cfg.nnx, cfg.nny = int(grid.nnx), int(grid.nny)
+        cfg.nnz = int(grid.nnz) if grid.dim == 3 else 1
+        cfg.x1, cfg.y1 = float(grid.x1), float(grid.y1)
+        cfg.dx, cfg.dy = float(grid.dx), float(grid.dy)
+        cfg.lpx, cfg.lpy = float(mp.dx), float(mp.dy)
+        if grid.dim == 3:
+            cfg.z1, cfg.dz, cfg.lpz = float(grid.z1), float(grid.dz), float(mp.dz)
+        else:
+            cfg.z1, cfg.dz, cfg.lpz = 0.0, 1.0, 1.0
+        self.cfg = cfg
+        ctx = capi.C.c_void_p()
+        rc = lib.mpm_create(capi.C.byref(cfg), capi.C.byref(ctx))
+        if rc != capi.MPM_OK:
+            raise capi.MPMError(rc, lib.mpm_last_error(None).decode())
+        self.ctx = ctx
+        try:
+            self._upload(grid, mp, attr, bc)
+        except Exception:
+            self.close()
+            raise
+
+    def _check(self, rc):
+        return capi.check(self.lib, self.ctx, rc)
+
+    def _upload(self, grid, mp, attr, bc):
+        f64 = lambda a: np.asfortranarray(np.asarray(a, dtype=np.float64))
+        i64 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+        n = self.np
+        arrs = dict(
+            xi=f64(mp.ξ), vs=f64(mp.vs), sigma=f64(mp.σij), eps_s=f64(mp.ϵijs), F=f64(mp.F),
+            omega=f64(mp.Ω), omega0=f64(mp.Ω0), rho=f64(mp.ρs), rho0=f64(mp.ρs0),
+            eps_q=f64(mp.ϵq), eps_k=f64(mp.ϵk), nid=i64(attr.nid))
+        if arrs["nid"].size != n:
+            raise ValueError("attr.nid must have one entry per particle")
+        d = capi.dptr
+        self._check(self.lib.mpm_upload_particles(
+            self.ctx, d(arrs["xi"]), d(arrs["vs"]), d(arrs["sigma"]), d(arrs["eps_s"]),
+            d(arrs["F"]), d(arrs["omega"]), d(arrs["omega0"]), d(arrs["rho"]), d(arrs["rho0"]),
+            d(arrs["eps_q"]), d(arrs["eps_k"]), capi.iptr(arrs["nid"])))
+        m = [f64(a) for a in (attr.Ks, attr.Gs, attr.ϕ, attr.ψ, attr.c, attr.σt, attr.Hp, attr.cr)]
+        self._check(self.lib.mpm_upload_materials(self.ctx, *[capi.dptr(a) for a in m]))
+        ni = int(grid.ni)
+        bx = [i64(bc.vx_s_idx), f64(bc.vx_s_val), i64(bc.vy_s_idx), f64(bc.vy_s_val)]
+        if self.dim == 3:
+            bx += [i64(bc.vz_s_idx), f64(bc.vz_s_val)]
+        for a in bx:
+            if a.size != ni:
+                raise ValueError(f"boundary arrays must have grid.ni = {ni} entries")
+        if self.dim == 2:
+            bx += [None, None]
+        self._check(self.lib.mpm_upload_bc(
+            self.ctx, capi.iptr(bx[0]), capi.dptr(bx[1]), capi.iptr(bx[2]), capi.dptr(bx[3]),
+            capi.iptr(bx[4]), capi.dptr(bx[5])))
+
+    @staticmethod
+    def step_args(args):
+        a = capi.MPMArgs()
+        if args.scheme != "MUSL" or args.coupling != "OS":
+            raise NotImplementedError("the HIP path implements coupling=:OS, scheme=:MUSL")
+        if args.constitutive not in capi.CONSTITUTIVE:
+            raise NotImplementedError(f"constitutive {args.constitutive} has no HIP kernels")
+        a.scheme = capi.MPM_MUSL
+        a.va = capi.MPM_VA_A if args.va == "a" else capi.MPM_VA_V
+        a.constitutive = capi.CONSTITUTIVE[args.constitutive]
+        a.mvl = 1 if args.MVL else 0
+        a.flip, a.pic = float(args.FLIP), float(args.PIC)
+        a.zeta_s, a.gravity, a.Te = float(args.ζs), float(args.gravity), float(args.Te)
+        return a
+
+    def step(self, sargs, dt, Ti):
+        self._check(self.lib.mpm_step(self.ctx, capi.C.byref(sargs), float(dt), float(Ti)))
+
+    def run(self, sargs, dt, Ti, nsteps):
+        out = capi.C.c_double(0.0)
+        self._check(self.lib.mpm_run(self.ctx, capi.C.byref(sargs), float(dt), float(Ti),
+                                     int(nsteps), capi.C.byref(out)))
+        return out.value
+
+    def sync(self):
+        self._check(self.lib.mpm_sync(self.ctx))
+
+    def download(self):
+        """The device2host! field set in the caller's particle order (FP64), keyed
+        xi, vs, sigma, sigma_m, omega, eps_q, eps_k, eps_v, eps_s, ms."""
+        n, D = self.np, self.dim
+        ns = 4 if D == 2 else 6
+        out = dict(xi=np.zeros((n, D), order="F"), vs=np.zeros((n, D), order="F"),
+                   sigma=np.zeros((n, ns), order="F"), sigma_m=np.zeros(n), omega=np.zeros(n),
+                   eps_q=np.zeros(n), eps_k=np.zeros(n), eps_v=np.zeros(n),
+                   eps_s=np.zeros((n, ns), order="F"), ms=np.zeros(n))
+        p = capi.dptr
+        self._check(self.lib.mpm_download_particles(
+            self.ctx, p(out["xi"]), p(out["vs"]), p(out["sigma"]), p(out["sigma_m"]),
+            p(out["omega"]), p(out["eps_q"]), p(out["eps_k"]), p(out["eps_v"]),
+            p(out["eps_s"]), p(out["ms"])))
+        return out
+
+    def count_active_nodes(self):
+        out = np.zeros(1, dtype=np.int64)
+        self._check(self.lib.mpm_count_active_nodes(self.ctx, capi.iptr(out)))
+        return int(out[0])
+
+    def profile_enable(self, on=True):
+        self._check(self.lib.mpm_profile_enable(self.ctx, 1 if on else 0))
+
+    def profile_read(self):
+        k = self.lib.mpm_profile_kernel_count()
+        ms = np.zeros(k)
+        cnt = np.zeros(k, dtype=np.int64)
+        self._check(self.lib.mpm_profile_read(self.ctx, capi.dptr(ms), capi.iptr(cnt), k))
+        names = [self.lib.mpm_profile_kernel_name(i).decode() for i in range(k)]
+        return {names[i]: (float(ms[i]), int(cnt[i])) for i in range(k)}
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.mpm_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _DeviceHandle:
+    """dev_grid / dev_mp / dev_attr / dev_bc returned by host2device: views of one model."""
+
+    def __init__(self, kind, model, host):
+        self.kind, self.model, self.host = kind, model, host
+
+    def __getattr__(self, name):
+        return getattr(self.host, name)
+
+
+def _require_hip(device):
+    if device != "HIP":
+        raise NotImplementedError(
+            f"device :{device} is not provided by this package; it implements the :HIP device "
+            "(libmpm_hip.so on MI355X).  There is no CPU fallback.")
+
+
+def host2device(grid, mp, attr, bc, device, sort_interval=None):
+    """host2device(grid, mp, attr, bc, Val(:HIP)) -- ext/AMDExt/devicehelpfunc_amd.jl:17-37."""
+    _require_hip(device)
+    if sort_interval is None:
+        sort_interval = 10 if mp.np >= 65536 else 0
+    model = HIPModel(grid, mp, attr, bc, sort_interval=sort_interval)
+    return (_DeviceHandle("grid", model, grid), _DeviceHandle("mp", model, mp),
+            _DeviceHandle("attr", model, attr), _DeviceHandle("bc", model, bc))
+
+
+def device2host(args, mp, dev_mp, device, verbose=False):
+    """device2host!(args, mp, dev_mp, Val(:HIP)) -- devicehelpfunc_amd.jl:39-69: copies
+    σm, vs, ξ, Ω, σij, ϵk, ϵq, ϵv, ϵijs, ms into the host struct in place."""
+    _require_hip(device)
+    out = dev_mp.model.download()
+    pairs = ((mp.σm, "sigma_m"), (mp.vs, "vs"), (mp.ξ, "xi"), (mp.Ω, "omega"),
+             (mp.σij, "sigma"), (mp.ϵk, "eps_k"), (mp.ϵq, "eps_q"), (mp.ϵv, "eps_v"),
+             (mp.ϵijs, "eps_s"), (mp.ms, "ms"))
+    for dst, key in pairs:
+        dst[...] = out[key].astype(dst.dtype)
+    if verbose:
+        print("[▼ I/O: downloading from :HIP → host]")
+
+
+def clean_device(dev_grid, dev_mp, dev_attr, dev_bc, device):
+    """clean_device!(..., Val(:HIP)) -- devicehelpfunc_amd.jl:71-102."""
+    _require_hip(device)
+    dev_mp.model.close()
+
+
+def getBackend(device):
+    """getBackend(Val(:HIP)): the synchronisation handle of the device."""
+    _require_hip(device)
+    return "HIP"
+
+
+def KAsync(dev_handle):
+    """KAsync(getBackend(Val(:HIP))) -- wait for the device, report kernel flags."""
+    dev_handle.model.sync()
+
+
+def procedure(args, grid, mp, attr, bc, ΔT, Ti, coupling, scheme, va):
+    """procedure!(args, grid, mp, attr, bc, ΔT, Ti, Val(:OS), Val(:MUSL), Val(va))
+    -- src/solvers/OS_MUSL.jl:12-102, as one device step."""
+    if coupling != "OS" or scheme != "MUSL":
+        raise NotImplementedError("the HIP path implements Val(:OS), Val(:MUSL)")
+    if not isinstance(mp, _DeviceHandle):
+        raise TypeError("procedure needs the device structs returned by host2device")
+    sargs = HIPModel.step_args(args)
+    sargs.va = 0 if va == "a" else 1
+    mp.model.step(sargs, ΔT, Ti)
+
+
+def submit_work(args, grid, mp, attr, bc, workflow):
+    """submit_work! (src/solver.jl:37-131) for the HIP device: upload, the
+    ``while Ti < Ttol`` loop with ``Ti += ΔT`` in T2, final sync, download, free.
+    (The reference's host-side initmpstatus! only fills the per-slot basis
+    arrays, which the device recomputes every step.)"""
+    if args.hdf5:
+        raise NotImplementedError("HDF5 snapshots are not implemented in this package yet")
+    T2 = args.T2
+    Ti = T2(0.0)
+    ΔT = T2(args.ΔT)
+    dev_grid, dev_mp, dev_attr, dev_bc = host2device(grid, mp, attr, bc, args.device,
+                                                     sort_interval=args.sort_interval)
+    try:
+        args.start_time = time.time()
+        while Ti < args.Ttol:
+            workflow(args, dev_grid, dev_mp, dev_attr, dev_bc, ΔT, Ti, args.coupling,
+                     args.scheme, args.va)
+            Ti = T2(Ti + ΔT)
+            args.iter_num += 1
+        KAsync(dev_mp)
+        args.end_time = time.time()
+        device2host(args, mp, dev_mp, args.device, verbose=args.progressbar)
+    finally:
+        clean_device(dev_grid, dev_mp, dev_attr, dev_bc, args.device)
+
+
+def info_print(args, grid, mp):
+    if args.progressbar:
+        print(f"[ MaterialPointSolver (HIP) ] {args.project_name}: {args.constitutive}, "
+              f"np={mp.np}, ni={grid.ni}, ϵ={args.ϵ}")
+
+
+def perf(args):
+    """perf (src/toolkits/terminaltxt.jl:67-85): iterations per second of the run."""
+    wall = max(args.end_time - args.start_time, 1e-12)
+    it_s = args.iter_num / wall
+    if args.progressbar:
+        print(f"[ Performance ] iterations: {args.iter_num}, wall time: {wall:.3f} s, "
+              f"{it_s:.2f} it/s")
+    return it_s
+
+
+def materialpointsolver(args, grid, mp, attr, bc, workflow=procedure):
+    """materialpointsolver! (src/MaterialPointSolver.jl:61-75)."""
+    if args.animation:
+        raise NotImplementedError("animation output is not implemented in this package yet")
+    info_print(args, grid, mp)
+    submit_work(args, grid, mp, attr, bc, workflow)
+    perf(args)
+
+
+def Tpeak(device="HIP", datatype="FP32", ID=0, nbytes=1 << 30, reps=10):
+    """Tpeak(Val(:HIP)) -- triad A = B + C bandwidth (devicehelpfunc_amd.jl:104-142).
+    Returns GB/s (1e9 B/s); the reference prints GiB/s."""
+    _require_hip(device)
+    lib = capi.load()
+    out = capi.C.c_double(0.0)
+    capi.check(lib, None, lib.mpm_tpeak(int(nbytes), int(reps), capi.C.byref(out)))
+    return out.value
