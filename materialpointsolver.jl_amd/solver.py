@@ -156,7 +156,7 @@ class HIPModel:
         return {names[i]: (float(ms[i]), int(cnt[i])) for i in range(k)}
 
     def close(self):
-        if getattr(self, "ctx", None):
+        if getattr(self, "ctx", None) is not None:
             self.lib.mpm_destroy(self.ctx)
             self.ctx = None
 

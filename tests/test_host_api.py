@@ -1,0 +1,106 @@
+"""Host-side mirror of the reference API, checked without a GPU: constructors
+and their validation, node ordering, Julia ranges, and the exact C ABI call
+sequence of materialpointsolver (with a recording stand-in for the library)."""
+import numpy as np
+import pytest
+
+
+def test_julia_ranges(mpm):
+    r = mpm.julia_range(0.0, 0.1, 1.0)
+    assert len(r) == 11 and r[3] == 0.3          # Julia: (0:0.1:1)[4] == 0.3
+    g = mpm.julia_range(-0.025, 0.0025, 0.82)
+    assert len(g) == 339 and g[10] == 0.0
+    assert len(mpm.julia_range(0.0, 0.3, 1.0)) == 4
+
+
+def test_grid2d_node_order(mpm):
+    g = mpm.UserGrid2D(x1=0.0, x2=0.3, y1=0.0, y2=0.2, dx=0.1, dy=0.1)
+    assert (g.nnx, g.nny, g.ni, g.ncx, g.ncy, g.nc) == (4, 3, 12, 3, 2, 6)
+    # node n = row + (col - 1) * nny, rows from the top (grid.jl:92-100)
+    assert np.allclose(g.ξ[:3, 1], [0.2, 0.1, 0.0])
+    assert np.allclose(g.ξ[:, 0], np.repeat([0.0, 0.1, 0.2, 0.3], 3))
+    assert g.p2nD.shape == (16, 2) and g.p2nD[0, 0] == -3 and g.p2nD[15, 1] == -1
+
+
+def test_grid3d_node_order(mpm):
+    g = mpm.UserGrid3D(x1=0, x2=0.2, y1=0, y2=0.3, z1=0, z2=0.1, dx=0.1, dy=0.1, dz=0.1)
+    assert (g.nnx, g.nny, g.nnz, g.ni) == (3, 4, 2, 24)
+    n = lambda ix, iy, iz: iy + ix * g.nny + iz * g.nny * g.nnx  # 0-based (grid.jl:239-249)
+    for ix in range(3):
+        for iy in range(4):
+            for iz in range(2):
+                assert np.allclose(g.ξ[n(ix, iy, iz)], [0.1 * ix, 0.1 * iy, 0.1 * iz])
+    assert g.p2nD.shape == (64, 4)
+
+
+def test_meshbuilder_order(mpm):
+    p = mpm.meshbuilder(np.array([0.0, 1.0]), np.array([5.0, 6.0, 7.0]))
+    assert p.tolist() == [[0, 5], [0, 6], [0, 7], [1, 5], [1, 6], [1, 7]]
+    q = mpm.meshbuilder(np.array([0.0, 1.0]), np.array([5.0, 6.0]), np.array([8.0, 9.0]))
+    assert q[:4].tolist() == [[0, 5, 8], [0, 6, 8], [1, 5, 8], [1, 6, 8]]
+
+
+def test_args_validation(mpm):
+    base = dict(Ttol=1.0, ΔT=1e-3, constitutive="druckerprager", project_name="t",
+                project_path="/tmp")
+    mpm.UserArgs2D(**base)
+    for bad, msg in [(dict(FLIP=0.5), r"FLIP\+PIC"), (dict(ζs=2), "ζs"), (dict(Te=3.0), "Elastic"),
+                     (dict(basis="quad"), "basis"), (dict(va="x"), "velocity"),
+                     (dict(device="TPU"), "device"), (dict(ΔT=2.0), "Time step")]:
+        with pytest.raises(ValueError, match=msg):
+            mpm.UserArgs3D(**{**base, **bad})
+    a = mpm.UserArgs2D(**base, animation=True)
+    assert a.hdf5
+
+
+def test_particle_and_property(mpm):
+    mp = mpm.UserParticle3D(dx=0.1, dy=0.1, dz=0.2, ξ=np.zeros((5, 3)), ρs=np.full(5, 2.0))
+    assert mp.np == 5 and np.allclose(mp.Ω0, 0.002) and np.allclose(mp.F[:, [0, 4, 8]], 1)
+    with pytest.raises(ValueError):
+        mpm.UserParticle2D(dx=0.1, dy=0.1, ξ=np.zeros((5, 3)), ρs=[1.0])
+    with pytest.raises(ValueError, match="nid layer"):
+        mpm.UserProperty(nid=[1, 1, 2], ν=[0.3], Es=[1.0], Gs=[1.0], Ks=[1.0])
+    at = mpm.UserProperty(nid=[1, 2, 2], ν=[0.3, 0.2], Es=[1, 2], Gs=[1, 2], Ks=[1, 2], ϕ=[0.5])
+    assert at.nmat == 2 and np.allclose(at.ϕ, [0.5, 0.5])
+
+
+def test_non_hip_device_raises(mpm):
+    args, grid, mp, attr, bc = mpm.scenes.column_collapse_2d(h=0.02, nsteps=2)
+    with pytest.raises(NotImplementedError, match="HIP"):
+        mpm.host2device(grid, mp, attr, bc, "CPU")
+
+
+class _Recorder:
+    """Stand-in for libmpm_hip.so that records the ABI calls."""
+
+    def __init__(self):
+        self.calls = []
+
+    def __getattr__(self, name):
+        def f(*a):
+            self.calls.append((name, a))
+            if name == "mpm_last_error":
+                return b"ok"
+            if name == "mpm_abi_version":
+                return 1
+            return 0
+        return f
+
+
+def test_solver_call_sequence_and_clock(mpm, monkeypatch):
+    rec = _Recorder()
+    monkeypatch.setattr(mpm._capi, "_lib", rec)
+    args, grid, mp, attr, bc = mpm.scenes.column_collapse_2d(h=0.02, dt=2e-5, Ttol=1e-3, eps="FP32")
+    mpm.materialpointsolver(args, grid, mp, attr, bc)
+    names = [c[0] for c in rec.calls]
+    assert names[:4] == ["mpm_create", "mpm_upload_particles", "mpm_upload_materials", "mpm_upload_bc"]
+    steps = [c for c in rec.calls if c[0] == "mpm_step"]
+    # Ti accumulates in T2 = Float32, as in submit_work! (solver.jl:118-124)
+    Ti, n = np.float32(0), 0
+    while Ti < np.float32(1e-3):
+        Ti = np.float32(Ti + np.float32(2e-5))
+        n += 1
+    assert len(steps) == n == args.iter_num
+    assert [s[1][3] for s in steps[:3]] == [0.0, float(np.float32(2e-5)),
+                                            float(np.float32(np.float32(2e-5) * 2))]
+    assert names[-3:] == ["mpm_sync", "mpm_download_particles", "mpm_destroy"]
