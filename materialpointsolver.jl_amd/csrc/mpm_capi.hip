@@ -56,6 +56,7 @@ struct mpm_ctx {
     bool have_particles = false, have_materials = false, have_bc = false;
     long long steps = 0;
     bool prof_on = false;
+    bool use_tiles = true;  // LDS-binned scatters when the particles are tile-sorted
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> pool;
 
@@ -431,8 +432,15 @@ struct Impl : ImplBase {
         if (cf.sort_interval > 0 && (c->steps % cf.sort_interval) == 0) {
             if ((rc = sort.run(c, pp, s)) != MPM_OK) return rc;
         }
+        const bool tiled = sort.enabled && sort.tiles_valid && c->use_tiles;
+        constexpr int TSH = DIM == 3 ? 2 : 3;
         if ((rc = c->launch(KID_P2G, [&] {
-                 k_p2g<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, R(G), n, fl);
+                 if (tiled)
+                     k_p2g_tiled<DIM, S, TSH, R><<<sort.tiled_grid(), 64, 0, s>>>(
+                         geom, sort.tg, pp, gp, R(G), sort.tile_list, sort.seg_begin, sort.seg_end,
+                         sort.tile_count, fl);
+                 else
+                     k_p2g<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, R(G), n, fl);
              })) != MPM_OK)
             return rc;
         if ((rc = c->launch(KID_GRID1, [&] {
@@ -440,7 +448,12 @@ struct Impl : ImplBase {
              })) != MPM_OK)
             return rc;
         if ((rc = c->launch(KID_G2P1, [&] {
-                 k_g2p1_p2g2<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, sa, n, fl);
+                 if (tiled)
+                     k_g2p1_p2g2_tiled<DIM, S, TSH, R><<<sort.tiled_grid(), 64, 0, s>>>(
+                         geom, sort.tg, pp, gp, sa, sort.tile_list, sort.seg_begin, sort.seg_end,
+                         sort.tile_count);
+                 else
+                     k_g2p1_p2g2<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, sa, n, fl);
              })) != MPM_OK)
             return rc;
         if ((rc = c->launch(KID_GRID2, [&] {

@@ -27,7 +27,21 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 namespace mpm {
+
+// Compile-time loop: f(std::integral_constant<int, 0>) ... f(ic<N-1>).  The
+// stencil loops use it so their register arrays are always indexed with
+// constants (a loop the compiler declines to unroll would move them to scratch).
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 constexpr double kEpsF64 = 2.220446049250313e-16;  // eps(Float64): massless-node cut
 
@@ -222,24 +236,20 @@ template <int DIM, int S, typename R, typename F>
 __device__ __forceinline__ void for_stencil(const GridGeom& g, const Stencil<DIM, S, R>& st,
                                             F&& body) {
     if constexpr (DIM == 2) {
-#pragma unroll
-        for (int a = 0; a < S; ++a)
-#pragma unroll
-            for (int b = 0; b < S; ++b) {
+        static_for<S>([&](auto a) {
+            static_for<S>([&](auto b) {
                 R Nx = st.N[0][a], Ny = st.N[1][b];
                 R N = Nx * Ny;
                 R dNx = st.dN[0][a] * Ny;
                 R dNy = st.dN[1][b] * Nx;
                 long long n = node_id(g, st.base[0] + a, st.base[1] + b, 0);
                 body(n, N, dNx, dNy, R(0));
-            }
+            });
+        });
     } else {
-#pragma unroll
-        for (int c = 0; c < S; ++c)
-#pragma unroll
-            for (int b = 0; b < S; ++b)
-#pragma unroll
-                for (int a = 0; a < S; ++a) {
+        static_for<S>([&](auto c) {
+            static_for<S>([&](auto b) {
+                static_for<S>([&](auto a) {
                     R Nx = st.N[0][a], Ny = st.N[1][b], Nz = st.N[2][c];
                     R N = Nx * Ny * Nz;
                     R dNx = st.dN[0][a] * Ny * Nz;
@@ -247,7 +257,33 @@ __device__ __forceinline__ void for_stencil(const GridGeom& g, const Stencil<DIM
                     R dNz = st.dN[2][c] * Nx * Ny;
                     long long n = node_id(g, st.base[0] + a, st.base[1] + b, st.base[2] + c);
                     body(n, N, dNx, dNy, dNz);
-                }
+                });
+            });
+        });
+    }
+}
+
+// Same iteration as for_stencil, with the node given as an offset inside a
+// WW-wide local window whose corner is the stencil base.
+template <int DIM, int S, typename R, typename F>
+__device__ __forceinline__ void for_stencil_local(const Stencil<DIM, S, R>& st, int WW, F&& body) {
+    if constexpr (DIM == 2) {
+        static_for<S>([&](auto a) {
+            static_for<S>([&](auto b) {
+                R Nx = st.N[0][a], Ny = st.N[1][b];
+                body(b * WW + a, Nx * Ny, st.dN[0][a] * Ny, st.dN[1][b] * Nx, R(0));
+            });
+        });
+    } else {
+        static_for<S>([&](auto c) {
+            static_for<S>([&](auto b) {
+                static_for<S>([&](auto a) {
+                    R Nx = st.N[0][a], Ny = st.N[1][b], Nz = st.N[2][c];
+                    body((c * WW + b) * WW + a, Nx * Ny * Nz, st.dN[0][a] * Ny * Nz,
+                         st.dN[1][b] * Nx * Nz, st.dN[2][c] * Nx * Ny);
+                });
+            });
+        });
     }
 }
 

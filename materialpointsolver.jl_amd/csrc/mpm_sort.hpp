@@ -17,6 +17,7 @@
 
 #include "../../include/mpm_hip.h"
 #include "mpm_kernels.hpp"
+#include "mpm_tiled.hpp"
 
 namespace mpm {
 
@@ -50,8 +51,16 @@ __global__ __launch_bounds__(256) void k_sort_keys(GridGeom g, SortKeyGeom kg,
     unsigned tile = unsigned(cc[0] >> kg.tsh) +
                     unsigned(kg.tx) * (unsigned(cc[1] >> kg.tsh) + unsigned(kg.ty) * unsigned(cc[2] >> kg.tsh));
     unsigned local = unsigned(cc[0] & tm) + unsigned(te) * (unsigned(cc[1] & tm) + unsigned(te) * unsigned(cc[2] & tm));
-    unsigned per_tile = DIM == 3 ? unsigned(te * te * te) : unsigned(te * te);
-    keys[i] = tile * per_tile + local;
+    // quadrant of the particle inside its cell (upper half per axis): particles
+    // of one quadrant share the stencil offset, so ordering tiles by
+    // (quadrant, cell) gives 64 consecutive particles 64 distinct stencil bases
+    unsigned quad = 0;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+        double u = (xs[a][i] - g.org[a]) * g.hinv[a];
+        if (u - floor(u) >= 0.5) quad |= 1u << a;
+    }
+    keys[i] = (tile << (DIM + 6)) | (quad << 6) | local;
     idx[i] = i;
 }
 
@@ -71,6 +80,11 @@ struct SortState {
     SortKeyGeom kg{};
     GridGeom geom{};
     int key_bits = 32;
+    TileGeom tg{};
+    // tile segments of the sorted order (valid after the first sort)
+    int *seg_begin = nullptr, *seg_end = nullptr, *tile_list = nullptr, *tile_count = nullptr;
+    unsigned char* nonempty = nullptr;
+    bool tiles_valid = false;
     unsigned *keys = nullptr, *keys_alt = nullptr;
     int *idx = nullptr, *idx_alt = nullptr;
     void* temp = nullptr;
@@ -95,15 +109,24 @@ struct SortState {
         kg.tx = (kg.ncx + te - 1) / te;
         kg.ty = (kg.ncy + te - 1) / te;
         kg.tz = dim == 3 ? (kg.ncz + te - 1) / te : 1;
-        unsigned long long maxkey = (unsigned long long)kg.tx * kg.ty * kg.tz *
-                                    (dim == 3 ? te * te * te : te * te);
+        unsigned long long maxkey = (unsigned long long)kg.tx * kg.ty * kg.tz << (dim + 6);
         if (maxkey >= (1ull << 32)) {
             error = "grid too large for 32-bit sort keys";
             return MPM_E_ARG;
         }
         key_bits = 1;
         while ((1ull << key_bits) < maxkey) ++key_bits;
+        tg.tsh = kg.tsh;
+        tg.tx = kg.tx;
+        tg.ty = kg.ty;
+        tg.tz = kg.tz;
+        tg.ntiles = kg.tx * kg.ty * kg.tz;
         hipError_t e = hipMalloc((void**)&keys, sizeof(unsigned) * np);
+        if (e == hipSuccess) e = hipMalloc((void**)&seg_begin, sizeof(int) * tg.ntiles);
+        if (e == hipSuccess) e = hipMalloc((void**)&seg_end, sizeof(int) * tg.ntiles);
+        if (e == hipSuccess) e = hipMalloc((void**)&tile_list, sizeof(int) * tg.ntiles);
+        if (e == hipSuccess) e = hipMalloc((void**)&tile_count, sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void**)&nonempty, tg.ntiles);
         if (e == hipSuccess) e = hipMalloc((void**)&keys_alt, sizeof(unsigned) * np);
         if (e == hipSuccess) e = hipMalloc((void**)&idx, sizeof(int) * np);
         if (e == hipSuccess) e = hipMalloc((void**)&idx_alt, sizeof(int) * np);
@@ -122,7 +145,8 @@ struct SortState {
     }
 
     void release() {
-        void* ps[] = {keys, keys_alt, idx, idx_alt, temp};
+        void* ps[] = {keys, keys_alt, idx, idx_alt, temp, seg_begin, seg_end, tile_list,
+                      tile_count, nonempty};
         for (void* p : ps)
             if (p) hipFree(p);
         keys = keys_alt = nullptr;
@@ -181,7 +205,25 @@ struct SortState {
         });
         if (rc != MPM_OK) return rc;
         if (e != hipSuccess) return c->hip_fail(e, "k_permute");
+        // tile segments of the new order and the ordered list of non-empty tiles
+        rc = c->launch(6, [&] {
+            const unsigned tb = unsigned((tg.ntiles + 255) / 256);
+            k_fill_u8<<<tb, 256, 0, s>>>(nonempty, tg.ntiles);
+            k_tile_segments<<<nb, 256, 0, s>>>(keys_alt, n, dim + 6, seg_begin, seg_end, nonempty);
+            k_compact_tiles<<<1, 1024, 0, s>>>(nonempty, tg.ntiles, tile_list, tile_count);
+        });
+        if (rc != MPM_OK) return rc;
+        tiles_valid = true;
         return MPM_OK;
+    }
+
+    // Grid size of the tiled kernels (one wavefront per workgroup): a multiple
+    // of 8 so that each XCD gets the same share.
+    unsigned tiled_grid() const {
+        long long g = ((long long)tg.ntiles + 7) / 8 * 8;
+        if (g > 16384) g = 16384;
+        if (g < 8) g = 8;
+        return unsigned(g);
     }
 };
 

@@ -56,7 +56,8 @@ def test_hip_fp64_matches_golden(mpm, idx):
         model.close()
     for k in KEYS:
         ref1, ref100 = g["step1_" + k], g["step100_" + k]
-        n1 = max(np.linalg.norm(ref1), 1e-30)
-        n100 = max(np.linalg.norm(ref100), 1e-30)
-        assert np.linalg.norm(s1[k] - ref1) / n1 < 1e-10, k
-        assert np.linalg.norm(s100[k] - ref100) / n100 < 1e-9, k
+        # relative L2 with an absolute floor for fields that are zero up to
+        # rounding (e.g. ϵk ~ 1e-21 at the cohesionless apex)
+        floor = 1e-14 * np.sqrt(ref1.size)
+        assert np.linalg.norm(s1[k] - ref1) <= 1e-10 * np.linalg.norm(ref1) + floor, k
+        assert np.linalg.norm(s100[k] - ref100) <= 1e-9 * np.linalg.norm(ref100) + floor, k
