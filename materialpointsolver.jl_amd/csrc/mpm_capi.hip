@@ -55,6 +55,7 @@ struct mpm_ctx {
     int* d_flags = nullptr;
     bool have_particles = false, have_materials = false, have_bc = false;
     long long steps = 0;
+    bool resort = true;  // particle order changed outside the sort (upload)
     bool prof_on = false;
     bool use_tiles = true;  // LDS-binned scatters when the particles are tile-sorted
     std::vector<ProfRec> recs;
@@ -333,6 +334,8 @@ struct Impl : ImplBase {
         if (e == hipSuccess) e = hipMemset(pp.p2c, 0xff, sizeof(int) * np);  // -1: no cell
         for (int a = 0; a < dim && e == hipSuccess; ++a) e = hipMemset(pp.dxp[a], 0, sizeof(R) * np);
         if (e != hipSuccess) return c->hip_fail(e, "upload nid/pid");
+        sort.tiles_valid = false;  // caller order: sort again before the next step
+        c->resort = true;
         return MPM_OK;
     }
 
@@ -429,16 +432,17 @@ struct Impl : ImplBase {
         hipStream_t s = c->stream;
         int* fl = c->d_flags;
         int rc;
-        if (cf.sort_interval > 0 && (c->steps % cf.sort_interval) == 0) {
+        if (cf.sort_interval > 0 && (c->resort || (c->steps % cf.sort_interval) == 0)) {
             if ((rc = sort.run(c, pp, s)) != MPM_OK) return rc;
+            c->resort = false;
         }
         const bool tiled = sort.enabled && sort.tiles_valid && c->use_tiles;
         constexpr int TSH = DIM == 3 ? 2 : 3;
         if ((rc = c->launch(KID_P2G, [&] {
                  if (tiled)
                      k_p2g_tiled<DIM, S, TSH, R><<<sort.tiled_grid(), 64, 0, s>>>(
-                         geom, sort.tg, pp, gp, R(G), sort.tile_list, sort.seg_begin, sort.seg_end,
-                         sort.tile_count, fl);
+                         geom, sort.tg, pp, gp, R(G), sort.sorted_keys(), sort.tile_list,
+                         sort.seg_begin, sort.seg_end, sort.tile_count, fl);
                  else
                      k_p2g<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, R(G), n, fl);
              })) != MPM_OK)
@@ -450,8 +454,8 @@ struct Impl : ImplBase {
         if ((rc = c->launch(KID_G2P1, [&] {
                  if (tiled)
                      k_g2p1_p2g2_tiled<DIM, S, TSH, R><<<sort.tiled_grid(), 64, 0, s>>>(
-                         geom, sort.tg, pp, gp, sa, sort.tile_list, sort.seg_begin, sort.seg_end,
-                         sort.tile_count);
+                         geom, sort.tg, pp, gp, sa, sort.sorted_keys(), sort.tile_list,
+                         sort.seg_begin, sort.seg_end, sort.tile_count);
                  else
                      k_g2p1_p2g2<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, sa, n, fl);
              })) != MPM_OK)
@@ -461,7 +465,12 @@ struct Impl : ImplBase {
              })) != MPM_OK)
             return rc;
         if ((rc = c->launch(KID_G2P2, [&] {
-                 k_g2p2<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, d_mats, sa, n, fl);
+                 if (tiled)
+                     k_g2p2_tiled<DIM, S, TSH, R><<<sort.tiled_grid(), 64, 0, s>>>(
+                         geom, sort.tg, pp, gp, d_mats, sa, sort.tile_list, sort.seg_begin,
+                         sort.seg_end, sort.tile_count, fl);
+                 else
+                     k_g2p2<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, d_mats, sa, n, fl);
              })) != MPM_OK)
             return rc;
         if (ar.mvl) {

@@ -176,6 +176,15 @@ struct Stencil {
 // two corners of the reference's cell (fld on x and z, cld on y,
 // utils_OS.jl:124-127).  Returns false when the window leaves the grid or x is
 // not finite.
+// Lowest node of the S-node window on axis a, as a float (u = (x - x1) / h):
+// floor(u - 1/2) for uGIMP; for the linear basis the reference's cell corner
+// (fld on x and z, cld on y).
+template <int S>
+__device__ __forceinline__ double stencil_base_f(int a, double u) {
+    if constexpr (S == 3) return floor(u - 0.5);
+    else return (a == 1) ? ceil(u) - 1.0 : floor(u);
+}
+
 template <int DIM, int S, typename R>
 __device__ __forceinline__ bool make_stencil(const GridGeom& g, const double* x,
                                              Stencil<DIM, S, R>& st, int& flag) {
@@ -188,14 +197,12 @@ __device__ __forceinline__ bool make_stencil(const GridGeom& g, const double* x,
             return false;
         }
         int b;
-        if constexpr (S == 3) {
-            double fb = floor(u - 0.5);
-            if (fb < 0.0 || fb > double(g.nn[a] - 3)) { ok = false; b = 0; }
-            else b = int(fb);
+        const double fb = stencil_base_f<S>(a, u);
+        if (fb < 0.0 || fb > double(g.nn[a] - S)) {
+            ok = false;
+            b = 0;
         } else {
-            double fb = (a == 1) ? ceil(u) - 1.0 : floor(u);
-            if (fb < 0.0 || fb > double(g.nn[a] - 2)) { ok = false; b = 0; }
-            else b = int(fb);
+            b = int(fb);
         }
         st.base[a] = b;
 #pragma unroll
@@ -405,6 +412,41 @@ __global__ __launch_bounds__(256) void k_grid1(GridGeom g, GridPtrs<R> gp, StepA
 // ---------------------------------------------------------------------------
 // MUSL re-map (doublemapping1(_a)_OS! + doublemapping2_OS!, utils_OS.jl:698-959)
 // ---------------------------------------------------------------------------
+// One node's share of the re-map sums: A = Σ N vsT, and for :a B = Σ N as,
+// C = Σ N vs; for :v B = Σ N (vsT - vs), C = Σ N vsT.
+template <int DIM, typename R>
+__device__ __forceinline__ void g2p1_accumulate(int va, R N, const R* vT, const R* vA, const R* vS,
+                                                R* A, R* B, R* C) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        if (va == 0) {
+            A[d] += N * vT[d];
+            B[d] += N * vA[d];
+            C[d] += N * vS[d];
+        } else {
+            A[d] += N * vT[d];
+            B[d] += N * (vT[d] - vS[d]);
+            C[d] += N * vT[d];
+        }
+    }
+}
+
+// New particle velocity and position increment; returns the new momentum.
+template <int DIM, typename R>
+__device__ __forceinline__ void g2p1_commit(PartPtrs<R>& pp, const StepArgs<R>& sa, int i,
+                                            const R* A, const R* B, const R* C, R* pv) {
+    const R m = pp.ms[i];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        R v = pp.v[d][i];
+        R vn = (sa.va == 0) ? sa.flip * v + sa.pic * C[d] + B[d] * sa.dt
+                            : sa.flip * (v + B[d]) + sa.pic * C[d];
+        pp.v[d][i] = vn;
+        pp.dxp[d][i] = A[d] * sa.dt;
+        pv[d] = m * vn;
+    }
+}
+
 template <int DIM, int S, typename R>
 __global__ __launch_bounds__(256) void k_g2p1_p2g2(GridGeom g, PartPtrs<R> pp, GridPtrs<R> gp,
                                                    StepArgs<R> sa, int np, int* flags) {
@@ -422,32 +464,18 @@ __global__ __launch_bounds__(256) void k_g2p1_p2g2(GridGeom g, PartPtrs<R> pp, G
     R A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, C[3] = {0, 0, 0};
     for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
         if (N != R(0)) {
+            R vT[3], vA[3], vS[3];
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                R vT = gp.vsT[d][n];
-                if (sa.va == 0) {
-                    A[d] += N * vT;
-                    B[d] += N * gp.as[d][n];
-                    C[d] += N * gp.vs[d][n];
-                } else {
-                    A[d] += N * vT;
-                    B[d] += N * (vT - gp.vs[d][n]);
-                    C[d] += N * vT;
-                }
+                vT[d] = gp.vsT[d][n];
+                vA[d] = gp.as[d][n];
+                vS[d] = gp.vs[d][n];
             }
+            g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
         }
     });
-    const R m = pp.ms[i];
     R pv[3];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        R v = pp.v[d][i];
-        R vn = (sa.va == 0) ? sa.flip * v + sa.pic * C[d] + B[d] * sa.dt
-                            : sa.flip * (v + B[d]) + sa.pic * C[d];
-        pp.v[d][i] = vn;
-        pp.dxp[d][i] = A[d] * sa.dt;
-        pv[d] = m * vn;
-    }
+    g2p1_commit<DIM, R>(pp, sa, i, A, B, C, pv);
     for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
         if (N != R(0)) {
 #pragma unroll
@@ -609,45 +637,15 @@ __device__ __forceinline__ int cell_of(const GridGeom& g, const double* x) {
     return cx + g.nc[0] * (cy + g.nc[1] * cz);
 }
 
-// ---------------------------------------------------------------------------
-// G2P #2 + constitutive + commit (G2P_OS!, utils_OS.jl:1025-1153; liE!; dpP!;
-// mcP!; vollock1_OS!)
-// ---------------------------------------------------------------------------
-template <int DIM, int S, typename R>
-__global__ __launch_bounds__(256) void k_g2p2(GridGeom g, PartPtrs<R> pp, GridPtrs<R> gp,
-                                              const Material<R>* __restrict__ mats,
-                                              StepArgs<R> sa, int np, int* flags) {
+// Everything G2P #2 does after gathering ΔF (row-major, DIM x DIM): strain
+// rate and increment, J, Ω, the constitutive update, MVL's cell sums and the
+// position commit (G2P_OS!, utils_OS.jl:1025-1153; liE!; dpP!; mcP!;
+// vollock1_OS!).  x is the step-start position.
+template <int DIM, typename R>
+__device__ __forceinline__ void g2p2_commit(const GridGeom& g, PartPtrs<R>& pp, GridPtrs<R>& gp,
+                                            const Material<R>* __restrict__ mats,
+                                            const StepArgs<R>& sa, int i, const double* x, R* F) {
     constexpr int NS = DIM == 2 ? 4 : 6;
-    constexpr int NF = DIM * DIM;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    double x[3];
-    load_x<DIM>(pp.x, i, x);
-    Stencil<DIM, S, R> st;
-    int flag = 0;
-    if (!make_stencil<DIM, S, R>(g, x, st, flag)) {
-        if (sa.mvl) pp.p2c[i] = -1;  // frozen particle: no cell
-        return;
-    }
-    R F[NF];
-#pragma unroll
-    for (int k = 0; k < NF; ++k) F[k] = R(0);
-    for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R dNx, R dNy, R dNz) {
-        if (N != R(0)) {
-            if constexpr (DIM == 2) {
-                R u0 = gp.du[0][n], u1 = gp.du[1][n];
-                F[0] += u0 * dNx;
-                F[1] += u0 * dNy;
-                F[2] += u1 * dNx;
-                F[3] += u1 * dNy;
-            } else {
-                R u0 = gp.du[0][n], u1 = gp.du[1][n], u2 = gp.du[2][n];
-                F[0] += u0 * dNx; F[1] += u0 * dNy; F[2] += u0 * dNz;
-                F[3] += u1 * dNx; F[4] += u1 * dNy; F[5] += u1 * dNz;
-                F[6] += u2 * dNx; F[7] += u2 * dNy; F[8] += u2 * dNz;
-            }
-        }
-    });
     const R rdt = R(1) / sa.dt;
     R e[NS];
     double detI;
@@ -717,6 +715,48 @@ __global__ __launch_bounds__(256) void k_g2p2(GridGeom g, PartPtrs<R> pp, GridPt
     }
 #pragma unroll
     for (int d = 0; d < DIM; ++d) pp.x[d][i] = x[d] + double(pp.dxp[d][i]);
+}
+
+// ---------------------------------------------------------------------------
+// G2P #2 + constitutive + commit (G2P_OS!, utils_OS.jl:1025-1153; liE!; dpP!;
+// mcP!; vollock1_OS!)
+// ---------------------------------------------------------------------------
+template <int DIM, int S, typename R>
+__global__ __launch_bounds__(256) void k_g2p2(GridGeom g, PartPtrs<R> pp, GridPtrs<R> gp,
+                                              const Material<R>* __restrict__ mats,
+                                              StepArgs<R> sa, int np, int* flags) {
+    constexpr int NS = DIM == 2 ? 4 : 6;
+    constexpr int NF = DIM * DIM;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    double x[3];
+    load_x<DIM>(pp.x, i, x);
+    Stencil<DIM, S, R> st;
+    int flag = 0;
+    if (!make_stencil<DIM, S, R>(g, x, st, flag)) {
+        if (sa.mvl) pp.p2c[i] = -1;  // frozen particle: no cell
+        return;
+    }
+    R F[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) F[k] = R(0);
+    for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R dNx, R dNy, R dNz) {
+        if (N != R(0)) {
+            if constexpr (DIM == 2) {
+                R u0 = gp.du[0][n], u1 = gp.du[1][n];
+                F[0] += u0 * dNx;
+                F[1] += u0 * dNy;
+                F[2] += u1 * dNx;
+                F[3] += u1 * dNy;
+            } else {
+                R u0 = gp.du[0][n], u1 = gp.du[1][n], u2 = gp.du[2][n];
+                F[0] += u0 * dNx; F[1] += u0 * dNy; F[2] += u0 * dNz;
+                F[3] += u1 * dNx; F[4] += u1 * dNy; F[5] += u1 * dNz;
+                F[6] += u2 * dNx; F[7] += u2 * dNy; F[8] += u2 * dNz;
+            }
+        }
+    });
+    g2p2_commit<DIM, R>(g, pp, gp, mats, sa, i, x, F);
     raise_flag(flags, flag);
 }
 

@@ -1,13 +1,21 @@
-// mpm_sort.hpp -- periodic on-device cell sort of the particle streams.
+// mpm_sort.hpp -- periodic on-device sort of the particle streams by stencil base.
 //
-// Every sort_interval steps the particles are reordered by a tile-blocked cell
-// key (tiles of 4x4x4 cells in 3D, 8x8 in 2D; x fastest inside a tile and
-// between tiles), so that consecutive lanes of a wavefront touch the same or
-// neighbouring grid nodes in every scatter and gather.  The key sort is the
-// rocPRIM radix sort (through hipCUB) over (key, index) pairs; the permutation
-// is one gather kernel per particle stream into a scratch buffer of the same
-// element size, followed by a pointer swap.  The original particle index
-// travels with the particles (pid) so downloads return the caller's order.
+// A particle's stencil base is the lowest node of its S-node window per axis
+// (floor(u - 1/2) for uGIMP, the reference's cell corner for the linear
+// basis, u = (x - x1) / h); particles with the same base form a "dual cell"
+// and scatter to the same S^D nodes.  Every sort_interval steps the particles
+// are reordered by the key
+//
+//     (tile of the base) << 11 | (rank inside the dual cell) << 6 | (base inside the tile)
+//
+// with tiles of 4x4x4 bases in 3D (8x8 in 2D) and the rank counted with one
+// integer atomic per particle.  Inside a tile, the particles of one rank then
+// have pairwise distinct bases, which is what the tiled scatter kernels need
+// (mpm_tiled.hpp).  The key sort is the rocPRIM radix sort (through hipCUB)
+// over (key, index) pairs; the permutation is one gather kernel per particle
+// stream into a scratch stream of the same element size, followed by a
+// pointer swap.  The original particle index travels with the particles (pid)
+// so downloads return the caller's order.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -22,45 +30,39 @@
 namespace mpm {
 
 struct SortKeyGeom {
-    int ncx, ncy, ncz;   // cells per axis
-    int tx, ty, tz;      // tiles per axis
-    int tsh;             // log2 tile edge (2 in 3D, 3 in 2D)
+    int nb[3];   // stencil bases per axis (nodes - S + 1)
+    int tsh;     // log2 tile edge in bases
+    int tx, ty;  // tiles per axis (x, y)
 };
 
-template <int DIM>
+template <int DIM, int S>
 __global__ __launch_bounds__(256) void k_sort_keys(GridGeom g, SortKeyGeom kg,
                                                    const double* __restrict__ x0,
                                                    const double* __restrict__ x1,
                                                    const double* __restrict__ x2,
+                                                   unsigned* __restrict__ count,
                                                    unsigned* __restrict__ keys,
                                                    int* __restrict__ idx, int np) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const double* xs[3] = {x0, x1, x2};
-    const int ncv[3] = {kg.ncx, kg.ncy, kg.ncz};
-    int cc[3] = {0, 0, 0};
+    int b[3] = {0, 0, 0};
 #pragma unroll
     for (int a = 0; a < DIM; ++a) {
-        double u = (xs[a][i] - g.org[a]) * g.hinv[a];
-        double f = floor(u);
+        double f = stencil_base_f<S>(a, (xs[a][i] - g.org[a]) * g.hinv[a]);
         if (!(f >= 0.0)) f = 0.0;  // also maps NaN to 0
-        if (f > double(ncv[a] - 1)) f = double(ncv[a] - 1);
-        cc[a] = int(f);
+        if (f > double(kg.nb[a] - 1)) f = double(kg.nb[a] - 1);
+        b[a] = int(f);
     }
-    const int te = 1 << kg.tsh, tm = te - 1;
-    unsigned tile = unsigned(cc[0] >> kg.tsh) +
-                    unsigned(kg.tx) * (unsigned(cc[1] >> kg.tsh) + unsigned(kg.ty) * unsigned(cc[2] >> kg.tsh));
-    unsigned local = unsigned(cc[0] & tm) + unsigned(te) * (unsigned(cc[1] & tm) + unsigned(te) * unsigned(cc[2] & tm));
-    // quadrant of the particle inside its cell (upper half per axis): particles
-    // of one quadrant share the stencil offset, so ordering tiles by
-    // (quadrant, cell) gives 64 consecutive particles 64 distinct stencil bases
-    unsigned quad = 0;
-#pragma unroll
-    for (int a = 0; a < DIM; ++a) {
-        double u = (xs[a][i] - g.org[a]) * g.hinv[a];
-        if (u - floor(u) >= 0.5) quad |= 1u << a;
-    }
-    keys[i] = (tile << (DIM + 6)) | (quad << 6) | local;
+    const unsigned lin = unsigned(b[0]) + unsigned(kg.nb[0]) * (unsigned(b[1]) + unsigned(kg.nb[1]) * unsigned(b[2]));
+    unsigned rank = atomicAdd(&count[lin], 1u);
+    if (rank > (1u << kRankBits) - 1) rank = (1u << kRankBits) - 1;
+    const int tm = (1 << kg.tsh) - 1;
+    const unsigned tile = unsigned(b[0] >> kg.tsh) +
+                          unsigned(kg.tx) * (unsigned(b[1] >> kg.tsh) + unsigned(kg.ty) * unsigned(b[2] >> kg.tsh));
+    const unsigned local = unsigned(b[0] & tm) +
+                           unsigned(tm + 1) * (unsigned(b[1] & tm) + unsigned(tm + 1) * unsigned(b[2] & tm));
+    keys[i] = (tile << kTileShift) | (rank << kLocalBits) | local;
     idx[i] = i;
 }
 
@@ -71,21 +73,27 @@ __global__ __launch_bounds__(256) void k_gather(const T* __restrict__ src, const
     if (i < np) dst[i] = src[idx[i]];
 }
 
+__global__ __launch_bounds__(256) void k_fill_u32(unsigned* __restrict__ p, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
 template <typename R>
 struct SortState {
     std::string error;
     bool enabled = false;
-    int dim = 3;
+    int dim = 3, S = 3;
     long long np = 0;
+    long long nbases = 0;
     SortKeyGeom kg{};
     GridGeom geom{};
-    int key_bits = 32;
     TileGeom tg{};
+    int key_bits = 32;
     // tile segments of the sorted order (valid after the first sort)
     int *seg_begin = nullptr, *seg_end = nullptr, *tile_list = nullptr, *tile_count = nullptr;
     unsigned char* nonempty = nullptr;
     bool tiles_valid = false;
-    unsigned *keys = nullptr, *keys_alt = nullptr;
+    unsigned *keys = nullptr, *keys_alt = nullptr, *count = nullptr;
     int *idx = nullptr, *idx_alt = nullptr;
     void* temp = nullptr;
     size_t temp_bytes = 0;
@@ -95,21 +103,24 @@ struct SortState {
     void* scratch8 = nullptr;   // np * 8 (only when R is 4 bytes)
     void* scratch4 = nullptr;   // np * 4 (int streams, only when R is 8 bytes)
 
+    // Sorted keys of the current order (tile | rank | base), read by the tiled kernels.
+    const unsigned* sorted_keys() const { return keys_alt; }
+
     int init(const mpm_config& cfg, const GridGeom& g, long long n) {
         if (cfg.sort_interval <= 0) return MPM_OK;
         enabled = true;
         dim = cfg.dim;
+        S = cfg.basis == MPM_LINEAR ? 2 : 3;
         np = n;
         geom = g;
-        kg.ncx = g.nc[0];
-        kg.ncy = g.nc[1];
-        kg.ncz = dim == 3 ? g.nc[2] : 1;
         kg.tsh = dim == 3 ? 2 : 3;
         const int te = 1 << kg.tsh;
-        kg.tx = (kg.ncx + te - 1) / te;
-        kg.ty = (kg.ncy + te - 1) / te;
-        kg.tz = dim == 3 ? (kg.ncz + te - 1) / te : 1;
-        unsigned long long maxkey = (unsigned long long)kg.tx * kg.ty * kg.tz << (dim + 6);
+        for (int a = 0; a < 3; ++a) kg.nb[a] = a < dim ? g.nn[a] - S + 1 : 1;
+        kg.tx = (kg.nb[0] + te - 1) / te;
+        kg.ty = (kg.nb[1] + te - 1) / te;
+        const int tz = dim == 3 ? (kg.nb[2] + te - 1) / te : 1;
+        nbases = (long long)kg.nb[0] * kg.nb[1] * kg.nb[2];
+        unsigned long long maxkey = ((unsigned long long)kg.tx * kg.ty * tz) << kTileShift;
         if (maxkey >= (1ull << 32)) {
             error = "grid too large for 32-bit sort keys";
             return MPM_E_ARG;
@@ -119,9 +130,10 @@ struct SortState {
         tg.tsh = kg.tsh;
         tg.tx = kg.tx;
         tg.ty = kg.ty;
-        tg.tz = kg.tz;
-        tg.ntiles = kg.tx * kg.ty * kg.tz;
+        tg.tz = tz;
+        tg.ntiles = kg.tx * kg.ty * tz;
         hipError_t e = hipMalloc((void**)&keys, sizeof(unsigned) * np);
+        if (e == hipSuccess) e = hipMalloc((void**)&count, sizeof(unsigned) * nbases);
         if (e == hipSuccess) e = hipMalloc((void**)&seg_begin, sizeof(int) * tg.ntiles);
         if (e == hipSuccess) e = hipMalloc((void**)&seg_end, sizeof(int) * tg.ntiles);
         if (e == hipSuccess) e = hipMalloc((void**)&tile_list, sizeof(int) * tg.ntiles);
@@ -146,12 +158,14 @@ struct SortState {
 
     void release() {
         void* ps[] = {keys, keys_alt, idx, idx_alt, temp, seg_begin, seg_end, tile_list,
-                      tile_count, nonempty};
+                      tile_count, nonempty, count};
         for (void* p : ps)
             if (p) hipFree(p);
-        keys = keys_alt = nullptr;
+        keys = keys_alt = count = nullptr;
         idx = idx_alt = nullptr;
         temp = nullptr;
+        seg_begin = seg_end = tile_list = tile_count = nullptr;
+        nonempty = nullptr;
     }
 
     template <typename T>
@@ -166,18 +180,23 @@ struct SortState {
         return hipSuccess;
     }
 
-    // Sort and permute every particle stream.  `c` is the context (for the
-    // launch wrapper that times k_sort and k_permute).
+    // Sort and permute every particle stream, then rebuild the tile segments.
+    // `c` is the context (its launch wrapper times k_sort and k_permute).
     template <typename Ctx>
     int run(Ctx* c, PartPtrs<R>& pp, hipStream_t s) {
         if (!enabled) return MPM_OK;
         const unsigned nb = unsigned((np + 255) / 256);
         const int n = int(np);
         int rc = c->launch(6, [&] {
-            if (dim == 3)
-                k_sort_keys<3><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], pp.x[2], keys, idx, n);
-            else
-                k_sort_keys<2><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], pp.x[1], keys, idx, n);
+            k_fill_u32<<<unsigned((nbases + 255) / 256), 256, 0, s>>>(count, nbases);
+            const double* x2 = dim == 3 ? pp.x[2] : pp.x[1];
+            if (dim == 3) {
+                if (S == 3) k_sort_keys<3, 3><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
+                else k_sort_keys<3, 2><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
+            } else {
+                if (S == 3) k_sort_keys<2, 3><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
+                else k_sort_keys<2, 2><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
+            }
             hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_alt, idx, idx_alt, n, 0,
                                                key_bits, s);
         });
@@ -209,7 +228,7 @@ struct SortState {
         rc = c->launch(6, [&] {
             const unsigned tb = unsigned((tg.ntiles + 255) / 256);
             k_fill_u8<<<tb, 256, 0, s>>>(nonempty, tg.ntiles);
-            k_tile_segments<<<nb, 256, 0, s>>>(keys_alt, n, dim + 6, seg_begin, seg_end, nonempty);
+            k_tile_segments<<<nb, 256, 0, s>>>(keys_alt, n, kTileShift, seg_begin, seg_end, nonempty);
             k_compact_tiles<<<1, 1024, 0, s>>>(nonempty, tg.ntiles, tile_list, tile_count);
         });
         if (rc != MPM_OK) return rc;

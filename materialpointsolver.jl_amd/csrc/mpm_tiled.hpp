@@ -1,21 +1,30 @@
-// mpm_tiled.hpp -- LDS-binned scatter kernels for cell-sorted particles.
+// mpm_tiled.hpp -- tile kernels for base-sorted particles (LDS windows).
 //
-// After every cell sort the particles of each tile (4x4x4 cells in 3D, 8x8 in
-// 2D) are contiguous.  A wavefront takes one tile at a time, accumulates the
-// scatter of its particles into an LDS window covering every node their
-// stencils can reach (tile edge + 3 nodes per axis), and then adds each touched
-// window node to the grid with one global atomic per field, instead of one per
-// particle-node-field.  Particles that drifted out of their tile's window
-// since the last sort scatter straight to the grid, so the result does not
-// depend on the sort interval.
+// After every sort (mpm_sort.hpp) the particles of each tile of stencil
+// bases (4x4x4 in 3D, 8x8 in 2D) are contiguous, ordered by (rank in the
+// dual cell, base).  One wavefront owns one tile at a time and an LDS window
+// of the grid nodes the tile's stencils reach, with one base of slack on every
+// side for particles that drifted since the sort (window edge = tile edge + 2
+// + S - 1 nodes):
 //
-// Inside a tile the particles are ordered by (quadrant of the cell, cell), so
-// 64 consecutive particles (one coalesced wavefront round) usually have 64
-// distinct stencil bases.
+//  * scatters (P2G #1, P2G #2) accumulate into the window and flush it with
+//    one global atomic per touched node and field;
+//  * gathers (the MUSL re-map, G2P #2) first stage the grid fields of the
+//    window in LDS and then read every stencil from there.
+//
+// Scatter rounds.  A round is at most 64 particles of one rank of one tile,
+// so at the last sort their bases were pairwise distinct.  Each lane claims
+// its current base in a small LDS table (write a unique tag, read it back) to
+// confirm that this still holds; the winners' targets are then distinct at
+// every stencil slot and they add into the window with plain LDS
+// read-add-write (volatile, so the slots stay in program order).  LDS float
+// atomics would serialise per lane on this hardware.  A particle that lost its
+// claim or drifted out of the window scatters straight to the grid with
+// global atomics, so results never depend on the sort interval.
 //
 // Tile entries are dealt to the 8 XCDs in contiguous chunks (a workgroup's
-// XCD is its blockIdx modulo 8), so neighbouring tiles, which share halo nodes
-// and gather the same grid values, run behind the same L2.
+// XCD is its blockIdx modulo 8), so neighbouring tiles, which share halo
+// nodes, run behind the same L2.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -23,10 +32,23 @@
 
 namespace mpm {
 
+constexpr int kRankBits = 5;   // ranks 0..31 (a fuller dual cell shares rank 31)
+constexpr int kLocalBits = 6;  // 64 bases per tile
+constexpr int kTileShift = kRankBits + kLocalBits;
+
 struct TileGeom {
-    int tsh;         // log2 of the tile edge in cells
+    int tsh;         // log2 of the tile edge in stencil bases
     int tx, ty, tz;  // tiles per axis
     int ntiles;
+};
+
+// Window geometry of a tile: NB bases and WW nodes per axis.
+template <int DIM, int S, int TSH>
+struct Win {
+    static constexpr int NB = (1 << TSH) + 2;
+    static constexpr int WW = NB + S - 1;
+    static constexpr int WN = DIM == 3 ? WW * WW * WW : WW * WW;
+    static constexpr int NBN = DIM == 3 ? NB * NB * NB : NB * NB;
 };
 
 __device__ __forceinline__ void xcd_chunk(int count, int& e0, int& e1, int& step) {
@@ -38,6 +60,7 @@ __device__ __forceinline__ void xcd_chunk(int count, int& e0, int& e1, int& step
     step = nloc;
 }
 
+// First window node of a tile (one base of slack below the tile).
 template <int DIM>
 __device__ __forceinline__ void tile_window_origin(const TileGeom& tg, int tile, int* w0) {
     const int cx = tile % tg.tx;
@@ -45,64 +68,132 @@ __device__ __forceinline__ void tile_window_origin(const TileGeom& tg, int tile,
     const int cz = DIM == 3 ? tile / (tg.tx * tg.ty) : 0;
     w0[0] = (cx << tg.tsh) - 1;
     w0[1] = (cy << tg.tsh) - 1;
-    w0[2] = (cz << tg.tsh) - 1;
+    w0[2] = DIM == 3 ? (cz << tg.tsh) - 1 : 0;
 }
 
-// Distinct-base rounds.  A wavefront holds one tile's LDS window.  In each
-// round its 64 lanes take 64 consecutive particles of the tile; a lane
-// "claims" its stencil base in a small LDS table and keeps the claim if no
-// other lane overwrote it.  The winners have pairwise distinct bases, so at
-// every stencil slot their target window nodes are distinct, and they add
-// into the window with plain LDS read-add-write (volatile, so the slots stay
-// in program order).  A lane that lost the claim (another particle of the
-// round shares its base, which the (quadrant, cell) order makes rare) scatters
-// straight to the grid.  This replaces LDS float atomics, which serialise per
-// lane on this hardware.
-template <int DIM, int S, int WW>
-struct WindowShape {
-    static constexpr int WN = DIM == 3 ? WW * WW * WW : WW * WW;  // window nodes
-    static constexpr int NB = WW - S + 1;                         // bases per axis
-    static constexpr int NBN = DIM == 3 ? NB * NB * NB : NB * NB;
-};
-
-__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
-
 // LDS-typed pointers: volatile accesses through a generic pointer would be
-// emitted as FLAT instructions, so the window is addressed in address space 3.
+// emitted as FLAT instructions, so the windows are addressed in address space 3.
 template <typename T>
 using lds_ptr = volatile __attribute__((address_space(3))) T*;
 
-// P2G #1 with LDS tile bins: m, p[DIM], f[DIM] (NF = 1 + 2*DIM fields).
+// Window offsets of a stencil base (node offset wb, base offset bidx); false
+// if the base is outside the window.
+template <int DIM, int NB, int WW>
+__device__ __forceinline__ bool window_slot(const int* base, const int* w0, int& wb, int& bidx) {
+    const int rx = base[0] - w0[0], ry = base[1] - w0[1], rz = DIM == 3 ? base[2] - w0[2] : 0;
+    const bool inside = rx >= 0 && rx < NB && ry >= 0 && ry < NB && rz >= 0 && rz < NB;
+    wb = DIM == 3 ? (rz * WW + ry) * WW + rx : ry * WW + rx;
+    bidx = DIM == 3 ? (rz * NB + ry) * NB + rx : ry * NB + rx;
+    return inside;
+}
+
+// Stage NF grid fields of a tile's window into LDS (zero outside the grid).
+template <int DIM, int WW, int WN, int NF, typename R>
+__device__ __forceinline__ void stage_window(const GridGeom& g, const int* w0, R* const* fields,
+                                             R* lds) {
+    for (int k = threadIdx.x; k < WN; k += 64) {
+        const int kx = k % WW, ky = (k / WW) % WW, kz = DIM == 3 ? k / (WW * WW) : 0;
+        const int gx = w0[0] + kx, gy = w0[1] + ky, gz = DIM == 3 ? w0[2] + kz : 0;
+        const bool in = gx >= 0 && gx < g.nn[0] && gy >= 0 && gy < g.nn[1] &&
+                        (DIM == 2 || (gz >= 0 && gz < g.nn[2]));
+        const long long n = in ? node_id(g, gx, gy, gz) : 0;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) lds[f * WN + k] = in ? fields[f][n] : R(0);
+    }
+}
+
+// Flush a scatter window: global atomic adds of every non-zero node (only
+// window nodes inside the grid can be non-zero).
+template <int DIM, int WW, int WN, int NF, typename R>
+__device__ __forceinline__ void flush_window(const GridGeom& g, const int* w0, lds_ptr<R> win,
+                                             R* const* fields) {
+    for (int k = threadIdx.x; k < WN; k += 64) {
+        R vals[NF];
+        bool any = false;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            vals[f] = win[f * WN + k];
+            any = any || vals[f] != R(0);
+        }
+        if (!any) continue;
+        const int kx = k % WW, ky = (k / WW) % WW, kz = DIM == 3 ? k / (WW * WW) : 0;
+        const long long n = node_id(g, w0[0] + kx, w0[1] + ky, DIM == 3 ? w0[2] + kz : 0);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) atomic_add(&fields[f][n], vals[f]);
+    }
+}
+
+// One scatter round from position `cur` of the sorted order: the particles
+// sharing the tile and rank of particle `cur` (a prefix of the next 64, since
+// the keys are sorted).  Returns their number; `active` tells whether this
+// lane's particle cur + lane is one of them.
+__device__ __forceinline__ int rank_round(const unsigned* __restrict__ keys, int cur, int end,
+                                          bool& active) {
+    const unsigned grp = keys[cur] >> kLocalBits;
+    const int q = cur + int(threadIdx.x);
+    active = q < end && (keys[q] >> kLocalBits) == grp;
+    return __popcll(__ballot(active));
+}
+
+// P2G #1 contribution of one particle to one node: m, p[DIM], f[DIM]
+// (utils_OS.jl:466-494).
+template <int DIM, typename R>
+__device__ __forceinline__ void p2g_values(R N, R dNx, R dNy, R dNz, R m, R vol, const R* pv,
+                                           const R* sg, R G, R* c) {
+    const R NiM = m * N;
+    c[0] = NiM;
+    c[1] = N * pv[0];
+    c[2] = N * pv[1];
+    if constexpr (DIM == 2) {
+        c[3] = -vol * (dNx * sg[0] + dNy * sg[3]);
+        c[4] = -vol * (dNy * sg[1] + dNx * sg[3]) + NiM * G;
+    } else {
+        c[3] = N * pv[2];
+        c[4] = -vol * (dNx * sg[0] + dNy * sg[3] + dNz * sg[5]);
+        c[5] = -vol * (dNy * sg[1] + dNx * sg[3] + dNz * sg[4]);
+        c[6] = -vol * (dNz * sg[2] + dNx * sg[5] + dNy * sg[4]) + NiM * G;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P2G #1 (P2G_OS!, utils_OS.jl:424-496)
+// ---------------------------------------------------------------------------
 template <int DIM, int S, int TSH, typename R>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_p2g_tiled(GridGeom g, TileGeom tg, PartPtrs<R> pp,
-                                                  GridPtrs<R> gp, R G,
-                                                  const int* __restrict__ tile_list,
-                                                  const int* __restrict__ seg_begin,
-                                                  const int* __restrict__ seg_end,
-                                                  const int* __restrict__ count_ptr, int* flags) {
-    constexpr int WW = (1 << TSH) + 3;
-    using WS = WindowShape<DIM, S, WW>;
-    constexpr int WN = WS::WN, NB = WS::NB;
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_p2g_tiled(
+    GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, R G,
+    const unsigned* __restrict__ keys, const int* __restrict__ tile_list,
+    const int* __restrict__ seg_begin, const int* __restrict__ seg_end,
+    const int* __restrict__ count_ptr, int* flags) {
+    using W = Win<DIM, S, TSH>;
+    constexpr int WW = W::WW, WN = W::WN, NB = W::NB;
     constexpr int NF = 1 + 2 * DIM;
     __shared__ R win_[NF * WN];
-    __shared__ unsigned claim_[WS::NBN];
+    __shared__ unsigned claim_[W::NBN];
     lds_ptr<R> win = (lds_ptr<R>)win_;
     lds_ptr<unsigned> claim = (lds_ptr<unsigned>)claim_;
-    const int lane = threadIdx.x;
-    unsigned tag = unsigned(lane);
+    R* fields[NF];
+    fields[0] = gp.m;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        fields[1 + d] = gp.p[d];
+        fields[1 + DIM + d] = gp.f[d];
+    }
+    unsigned tag = threadIdx.x;  // unique per lane and round
     int flag = 0;
     int e0, e1, est;
     xcd_chunk(*count_ptr, e0, e1, est);
     for (int e = e0; e < e1; e += est) {
         const int tile = tile_list[e];
-        const int beg = seg_begin[tile], P = seg_end[tile] - beg;
+        const int beg = seg_begin[tile], end = seg_end[tile];
         int w0[3];
         tile_window_origin<DIM>(tg, tile, w0);
-        for (int k = lane; k < NF * WN; k += 64) win[k] = R(0);
-        for (int r0 = 0; r0 < P; r0 += 64) {
-            const int q = r0 + lane;
-            if (q >= P) continue;
-            const int i = beg + q;
+        for (int k = threadIdx.x; k < NF * WN; k += 64) win[k] = R(0);
+        for (int cur = beg; cur < end;) {
+            bool active;
+            const int cnt = rank_round(keys, cur, end, active);
+            const int i = cur + int(threadIdx.x);
+            cur += cnt;
+            if (!active) continue;
             double x[3];
             load_x<DIM>(pp.x, i, x);
             Stencil<DIM, S, R> st;
@@ -119,35 +210,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 sg[4] = pp.sig[4][i];
                 sg[5] = pp.sig[5][i];
             }
-            const int rx = st.base[0] - w0[0], ry = st.base[1] - w0[1];
-            const int rz = DIM == 3 ? st.base[2] - w0[2] : 0;
-            const bool inside = rx >= 0 && rx < NB && ry >= 0 && ry < NB && rz >= 0 && rz < NB;
-            const int bidx = DIM == 3 ? (rz * NB + ry) * NB + rx : ry * NB + rx;
-            // distinct-base claim (see above); tags are unique per lane and round
+            int wb, bidx;
+            const bool inside = window_slot<DIM, NB, WW>(st.base, w0, wb, bidx);
             if (inside) claim[bidx] = tag;
             const bool won = inside && claim[bidx] == tag;
             tag += 64u;
-            auto contrib = [&](R N, R dNx, R dNy, R dNz, R* c) {
-                const R NiM = m * N;
-                c[0] = NiM;
-                c[1] = N * pv[0];
-                c[2] = N * pv[1];
-                if constexpr (DIM == 2) {
-                    c[3] = -vol * (dNx * sg[0] + dNy * sg[3]);
-                    c[4] = -vol * (dNy * sg[1] + dNx * sg[3]) + NiM * G;
-                } else {
-                    c[3] = N * pv[2];
-                    c[4] = -vol * (dNx * sg[0] + dNy * sg[3] + dNz * sg[5]);
-                    c[5] = -vol * (dNy * sg[1] + dNx * sg[3] + dNz * sg[4]);
-                    c[6] = -vol * (dNz * sg[2] + dNx * sg[5] + dNy * sg[4]) + NiM * G;
-                }
-            };
             if (won) {
-                const int wb = DIM == 3 ? (rz * WW + ry) * WW + rx : ry * WW + rx;
                 for_stencil_local<DIM, S, R>(st, WW, [&](int off, R N, R dNx, R dNy, R dNz) {
                     if (N != R(0)) {
                         R c[NF];
-                        contrib(N, dNx, dNy, dNz, c);
+                        p2g_values<DIM, R>(N, dNx, dNy, dNz, m, vol, pv, sg, G, c);
                         lds_ptr<R> L = win + wb + off;
                         R old[NF];
 #pragma unroll
@@ -159,120 +231,115 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     __builtin_amdgcn_sched_barrier(0);
                 });
             } else {
-                // drifted out of the tile's window since the last sort, or lost
-                // the claim to a particle with the same stencil base
+                // drifted out of the window, or shares its base with another
+                // particle of the round
                 for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R dNx, R dNy, R dNz) {
                     if (N != R(0)) {
                         R c[NF];
-                        contrib(N, dNx, dNy, dNz, c);
-                        atomic_add(&gp.m[n], c[0]);
+                        p2g_values<DIM, R>(N, dNx, dNy, dNz, m, vol, pv, sg, G, c);
 #pragma unroll
-                        for (int d = 0; d < DIM; ++d) atomic_add(&gp.p[d][n], c[1 + d]);
-#pragma unroll
-                        for (int d = 0; d < DIM; ++d) atomic_add(&gp.f[d][n], c[1 + DIM + d]);
+                        for (int f = 0; f < NF; ++f) atomic_add(&fields[f][n], c[f]);
                     }
                     __builtin_amdgcn_sched_barrier(0);
                 });
             }
         }
-        // flush the window: one global atomic per touched node and field
-        for (int k = lane; k < WN; k += 64) {
-            R vals[NF];
-            bool any = false;
-#pragma unroll
-            for (int f = 0; f < NF; ++f) {
-                vals[f] = win[f * WN + k];
-                any = any || vals[f] != R(0);
-            }
-            if (!any) continue;
-            const int kx = k % WW, ky = (k / WW) % WW, kz = DIM == 3 ? k / (WW * WW) : 0;
-            const long long n = node_id(g, w0[0] + kx, w0[1] + ky, DIM == 3 ? w0[2] + kz : 0);
-            atomic_add(&gp.m[n], vals[0]);
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) atomic_add(&gp.p[d][n], vals[1 + d]);
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) atomic_add(&gp.f[d][n], vals[1 + DIM + d]);
-        }
+        flush_window<DIM, WW, WN, NF, R>(g, w0, win, fields);
     }
     raise_flag(flags, flag);
 }
 
-// MUSL re-map with LDS tile bins for the momentum scatter (P2G #2).
+// ---------------------------------------------------------------------------
+// MUSL re-map: gather vsT, as, vs from the staged window, update the particle
+// velocity, scatter its new momentum into a second window
+// (doublemapping1(_a)_OS! + doublemapping2_OS!, utils_OS.jl:698-959)
+// ---------------------------------------------------------------------------
 template <int DIM, int S, int TSH, typename R>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p1_p2g2_tiled(GridGeom g, TileGeom tg, PartPtrs<R> pp,
-                                                        GridPtrs<R> gp, StepArgs<R> sa,
-                                                        const int* __restrict__ tile_list,
-                                                        const int* __restrict__ seg_begin,
-                                                        const int* __restrict__ seg_end,
-                                                        const int* __restrict__ count_ptr) {
-    constexpr int WW = (1 << TSH) + 3;
-    using WS = WindowShape<DIM, S, WW>;
-    constexpr int WN = WS::WN, NB = WS::NB;
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p1_p2g2_tiled(
+    GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, StepArgs<R> sa,
+    const unsigned* __restrict__ keys, const int* __restrict__ tile_list,
+    const int* __restrict__ seg_begin, const int* __restrict__ seg_end,
+    const int* __restrict__ count_ptr) {
+    using W = Win<DIM, S, TSH>;
+    constexpr int WW = W::WW, WN = W::WN, NB = W::NB;
+    constexpr int NG = 3 * DIM;  // gathered: vsT, as, vs
+    __shared__ R gwin[NG * WN];
     __shared__ R win_[DIM * WN];
-    __shared__ unsigned claim_[WS::NBN];
+    __shared__ unsigned claim_[W::NBN];
     lds_ptr<R> win = (lds_ptr<R>)win_;
     lds_ptr<unsigned> claim = (lds_ptr<unsigned>)claim_;
-    const int lane = threadIdx.x;
-    unsigned tag = unsigned(lane);
+    R* gfields[NG];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        gfields[d] = gp.vsT[d];
+        gfields[DIM + d] = gp.as[d];
+        gfields[2 * DIM + d] = gp.vs[d];
+    }
+    R* pfields[3] = {gp.p[0], gp.p[1], DIM == 3 ? gp.p[2] : gp.p[1]};
+    unsigned tag = threadIdx.x;
     int e0, e1, est;
     xcd_chunk(*count_ptr, e0, e1, est);
     for (int e = e0; e < e1; e += est) {
         const int tile = tile_list[e];
-        const int beg = seg_begin[tile], P = seg_end[tile] - beg;
+        const int beg = seg_begin[tile], end = seg_end[tile];
         int w0[3];
         tile_window_origin<DIM>(tg, tile, w0);
-        for (int k = lane; k < DIM * WN; k += 64) win[k] = R(0);
-        for (int r0 = 0; r0 < P; r0 += 64) {
-            const int q = r0 + lane;
-            if (q >= P) continue;
-            const int i = beg + q;
+        __syncthreads();  // the previous tile's reads of gwin are done
+        stage_window<DIM, WW, WN, NG, R>(g, w0, gfields, gwin);
+        for (int k = threadIdx.x; k < DIM * WN; k += 64) win[k] = R(0);
+        __syncthreads();
+        for (int cur = beg; cur < end;) {
+            bool active;
+            const int cnt = rank_round(keys, cur, end, active);
+            const int i = cur + int(threadIdx.x);
+            cur += cnt;
+            if (!active) continue;
             double x[3];
             load_x<DIM>(pp.x, i, x);
             Stencil<DIM, S, R> st;
-            int flag = 0;
+            int flag = 0;  // reported by k_p2g_tiled
             if (!make_stencil<DIM, S, R>(g, x, st, flag)) {
 #pragma unroll
                 for (int a = 0; a < DIM; ++a) pp.dxp[a][i] = R(0);
                 continue;
             }
+            int wb, bidx;
+            const bool inside = window_slot<DIM, NB, WW>(st.base, w0, wb, bidx);
             R A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, C[3] = {0, 0, 0};
-            for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
-                if (N != R(0)) {
+            if (inside) {
+                for_stencil_local<DIM, S, R>(st, WW, [&](int off, R N, R, R, R) {
+                    if (N != R(0)) {
+                        const R* L = gwin + wb + off;
+                        R vT[3], vA[3], vS[3];
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) {
-                        R vT = gp.vsT[d][n];
-                        if (sa.va == 0) {
-                            A[d] += N * vT;
-                            B[d] += N * gp.as[d][n];
-                            C[d] += N * gp.vs[d][n];
-                        } else {
-                            A[d] += N * vT;
-                            B[d] += N * (vT - gp.vs[d][n]);
-                            C[d] += N * vT;
+                        for (int d = 0; d < DIM; ++d) {
+                            vT[d] = L[d * WN];
+                            vA[d] = L[(DIM + d) * WN];
+                            vS[d] = L[(2 * DIM + d) * WN];
                         }
+                        g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
                     }
-                }
-            });
-            const R m = pp.ms[i];
-            R pv[3] = {0, 0, 0};
+                });
+            } else {
+                for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
+                    if (N != R(0)) {
+                        R vT[3], vA[3], vS[3];
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                R v = pp.v[d][i];
-                R vn = (sa.va == 0) ? sa.flip * v + sa.pic * C[d] + B[d] * sa.dt
-                                    : sa.flip * (v + B[d]) + sa.pic * C[d];
-                pp.v[d][i] = vn;
-                pp.dxp[d][i] = A[d] * sa.dt;
-                pv[d] = m * vn;
+                        for (int d = 0; d < DIM; ++d) {
+                            vT[d] = gp.vsT[d][n];
+                            vA[d] = gp.as[d][n];
+                            vS[d] = gp.vs[d][n];
+                        }
+                        g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
+                    }
+                });
             }
-            const int rx = st.base[0] - w0[0], ry = st.base[1] - w0[1];
-            const int rz = DIM == 3 ? st.base[2] - w0[2] : 0;
-            const bool inside = rx >= 0 && rx < NB && ry >= 0 && ry < NB && rz >= 0 && rz < NB;
-            const int bidx = DIM == 3 ? (rz * NB + ry) * NB + rx : ry * NB + rx;
+            R pv[3] = {0, 0, 0};
+            g2p1_commit<DIM, R>(pp, sa, i, A, B, C, pv);
             if (inside) claim[bidx] = tag;
             const bool won = inside && claim[bidx] == tag;
             tag += 64u;
             if (won) {
-                const int wb = DIM == 3 ? (rz * WW + ry) * WW + rx : ry * WW + rx;
                 for_stencil_local<DIM, S, R>(st, WW, [&](int off, R N, R, R, R) {
                     if (N != R(0)) {
                         lds_ptr<R> L = win + wb + off;
@@ -294,21 +361,83 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 });
             }
         }
-        for (int k = lane; k < WN; k += 64) {
-            R vals[3];
-            bool any = false;
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                vals[d] = win[d * WN + k];
-                any = any || vals[d] != R(0);
+        flush_window<DIM, WW, WN, DIM, R>(g, w0, win, pfields);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// G2P #2 with the Δu window staged in LDS (G2P_OS!, utils_OS.jl:1025-1153),
+// then the strain / constitutive / MVL / position commit of g2p2_commit
+// ---------------------------------------------------------------------------
+template <int DIM, int S, int TSH, typename R>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p2_tiled(
+    GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, const Material<R>* __restrict__ mats,
+    StepArgs<R> sa, const int* __restrict__ tile_list, const int* __restrict__ seg_begin,
+    const int* __restrict__ seg_end, const int* __restrict__ count_ptr, int* flags) {
+    using W = Win<DIM, S, TSH>;
+    constexpr int WW = W::WW, WN = W::WN, NB = W::NB;
+    constexpr int NF = DIM * DIM;
+    __shared__ R uwin[DIM * WN];
+    R* ufields[3] = {gp.du[0], gp.du[1], DIM == 3 ? gp.du[2] : gp.du[1]};
+    int flag = 0;
+    int e0, e1, est;
+    xcd_chunk(*count_ptr, e0, e1, est);
+    for (int e = e0; e < e1; e += est) {
+        const int tile = tile_list[e];
+        const int beg = seg_begin[tile], end = seg_end[tile];
+        int w0[3];
+        tile_window_origin<DIM>(tg, tile, w0);
+        __syncthreads();
+        stage_window<DIM, WW, WN, DIM, R>(g, w0, ufields, uwin);
+        __syncthreads();
+        for (int i = beg + int(threadIdx.x); i < end; i += 64) {
+            double x[3];
+            load_x<DIM>(pp.x, i, x);
+            Stencil<DIM, S, R> st;
+            if (!make_stencil<DIM, S, R>(g, x, st, flag)) {
+                if (sa.mvl) pp.p2c[i] = -1;  // frozen particle: no cell
+                continue;
             }
-            if (!any) continue;
-            const int kx = k % WW, ky = (k / WW) % WW, kz = DIM == 3 ? k / (WW * WW) : 0;
-            const long long n = node_id(g, w0[0] + kx, w0[1] + ky, DIM == 3 ? w0[2] + kz : 0);
+            int wb, bidx;
+            const bool inside = window_slot<DIM, NB, WW>(st.base, w0, wb, bidx);
+            R F[NF];
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) atomic_add(&gp.p[d][n], vals[d]);
+            for (int k = 0; k < NF; ++k) F[k] = R(0);
+            auto acc = [&](const R* u, R dNx, R dNy, R dNz) {
+                if constexpr (DIM == 2) {
+                    F[0] += u[0] * dNx;
+                    F[1] += u[0] * dNy;
+                    F[2] += u[1] * dNx;
+                    F[3] += u[1] * dNy;
+                } else {
+                    F[0] += u[0] * dNx; F[1] += u[0] * dNy; F[2] += u[0] * dNz;
+                    F[3] += u[1] * dNx; F[4] += u[1] * dNy; F[5] += u[1] * dNz;
+                    F[6] += u[2] * dNx; F[7] += u[2] * dNy; F[8] += u[2] * dNz;
+                }
+            };
+            if (inside) {
+                for_stencil_local<DIM, S, R>(st, WW, [&](int off, R N, R dNx, R dNy, R dNz) {
+                    if (N != R(0)) {
+                        R u[3];
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) u[d] = uwin[d * WN + wb + off];
+                        acc(u, dNx, dNy, dNz);
+                    }
+                });
+            } else {
+                for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R dNx, R dNy, R dNz) {
+                    if (N != R(0)) {
+                        R u[3];
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) u[d] = gp.du[d][n];
+                        acc(u, dNx, dNy, dNz);
+                    }
+                });
+            }
+            g2p2_commit<DIM, R>(g, pp, gp, mats, sa, i, x, F);
         }
     }
+    raise_flag(flags, flag);
 }
 
 // ---------------------------------------------------------------------------
