@@ -50,11 +50,104 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=6, help="oracle steps for the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(cores, 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc child runs (FETCH_SIZE, WRITE_SIZE)")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
+
+
+# Counter calibration: FETCH_SIZE / WRITE_SIZE of the streaming triad kernel
+# (a = b + s c over three float4 arrays of TRIAD_BYTES each: reads 2x, writes 1x).
+TRIAD_BYTES = 256 << 20
+KERNEL_RE = {
+    "k_p2g": r"mpm::k_p2g(_tiled)?<",
+    "k_grid1": r"mpm::k_grid1<",
+    "k_g2p1_p2g2": r"mpm::k_g2p1_p2g2(_tiled)?<",
+    "k_grid2": r"mpm::k_grid2<",
+    "k_g2p2": r"mpm::k_g2p2(_tiled)?<",
+}
+
+
+def pmc_child(a):
+    """Runs under rocprofv3 --pmc: a few steps of the workload, then the triad."""
+    import __graft_entry__ as ge
+
+    mpm = ge.load_package()
+    args, grid, mp, attr, bc = mpm.scenes.column_collapse_3d(h=a.h, eps=a.precision)
+    model = mpm.HIPModel(grid, mp, attr, bc, sort_interval=a.sort_interval)
+    sargs = model.step_args(args)
+    model.run(sargs, float(args.ΔT), 0.0, 14)
+    model.sync()
+    model.close()
+    print("# pmc child: 14 steps done", flush=True)
+    mpm.Tpeak(nbytes=TRIAD_BYTES, reps=2)
+    print("# pmc child: triad done", flush=True)
+
+
+def counter_means(csv_path):
+    """Mean counter value (KiB) per kernel-name regex over its dispatches."""
+    import csv
+    import re
+
+    vals = {}
+    with open(csv_path) as f:
+        for r in csv.DictReader(f):
+            vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    out = {}
+    pats = dict(KERNEL_RE, triad=r"k_triad")
+    for key, pat in pats.items():
+        xs = [v for name, vs in vals.items() if re.search(pat, name) for v in vs]
+        if xs:
+            xs.sort()
+            out[key] = xs[len(xs) // 2]  # median dispatch
+    return out
+
+
+def measure_traffic(a):
+    """HBM bytes per launch of each step kernel from two rocprofv3 --pmc child
+    runs (FETCH_SIZE, then WRITE_SIZE), each scaled by what the same counter
+    reads for the triad kernel of known size.  None when unavailable."""
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None, "rocprofv3 not found"
+    env = dict(os.environ, TMPDIR="/tmp")
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="mpm_pmc_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv",
+               "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__), "--pmc-child",
+               "--h", str(a.h), "--precision", a.precision, "--sort-interval", str(a.sort_interval)]
+        print(f"# traffic: rocprofv3 --pmc {counter}", file=sys.stderr, flush=True)
+        r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True)
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            shutil.rmtree(d, ignore_errors=True)
+            return None, f"{counter}: rc={r.returncode}, no counter file; " + r.stderr[-300:]
+        res[counter] = counter_means(files[0])
+        shutil.rmtree(d, ignore_errors=True)
+    fe, wr = res["FETCH_SIZE"], res["WRITE_SIZE"]
+    if "triad" not in fe or "triad" not in wr:
+        return None, "triad dispatch missing from the counter files"
+    f_fetch = 2 * TRIAD_BYTES / (fe["triad"] * 1024.0)
+    f_write = TRIAD_BYTES / (wr["triad"] * 1024.0)
+    out = {"fetch_scale": f_fetch, "write_scale": f_write, "kernels": {}}
+    for k in KERNEL_RE:
+        if k in fe and k in wr:
+            out["kernels"][k] = {"fetch_bytes": fe[k] * 1024.0 * f_fetch,
+                                 "write_bytes": wr[k] * 1024.0 * f_write}
+    return out, ""
 
 
 def main():
     a = parse()
+    if a.pmc_child:
+        pmc_child(a)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
               "multi-GPU sharding is described in DESIGN.md but not built.", file=sys.stderr)
@@ -111,6 +204,20 @@ def main():
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(a, args, grid, mp, attr, bc)
     model.close()
+    traffic, traffic_note = None, "--no-traffic"
+    if not a.no_traffic:
+        traffic, traffic_note = measure_traffic(a)
+        if traffic is None:
+            print(f"# traffic unavailable: {traffic_note}", file=sys.stderr, flush=True)
+    dom_traffic = None
+    if traffic and dom in traffic["kernels"]:
+        t = traffic["kernels"][dom]
+        dom_traffic = t["fetch_bytes"] + t["write_bytes"]
+        for k, t in traffic["kernels"].items():
+            print(f"#   traffic {k:14s} fetch {t['fetch_bytes'] / 1e6:8.1f} MB  write "
+                  f"{t['write_bytes'] / 1e6:8.1f} MB  (algorithmic "
+                  f"{kernels[k]['algo_bytes'] / 1e6 if k in kernels else 0:.1f} MB)",
+                  file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,
@@ -145,7 +252,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (dk["gbs"] / HBM_PEAK_GBS) if dk["gbs"] else None,
-            "traffic": None,
+            "traffic": dom_traffic,
+            "traffic_unit": "bytes per launch (FETCH_SIZE + WRITE_SIZE, triad-calibrated)",
+            "algorithmic_bytes": dk["algo_bytes"],
         },
         "cpu_baseline": cpu,
     }

@@ -360,6 +360,10 @@ __global__ __launch_bounds__(256) void k_grid1(GridGeom g, GridPtrs<R> gp, StepA
     }
     if (n >= g.ni) return;
     const R m = gp.m[n];
+    // A node without mass received nothing from P2G (every particle with a
+    // non-zero weight on it adds m_p N > 0), so p and f are still zero, and no
+    // gather reads it (gathers skip zero weights): nothing to do.
+    if (m == R(0)) return;
     const R msd = m < R(kEpsF64) ? R(0) : R(1) / m;
     const unsigned bc = gp.bcf[n];
     R P[3], Fc[3], vs[3];
@@ -492,6 +496,7 @@ __global__ __launch_bounds__(256) void k_grid2(GridGeom g, GridPtrs<R> gp, R dt)
     long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.ni) return;
     const R m = gp.m[n];
+    if (m == R(0)) return;  // no mass: p is zero and no G2P weight reads du here
     const R msd = m < R(kEpsF64) ? R(0) : R(1) / m;
     const unsigned bc = gp.bcf[n];
 #pragma unroll
