@@ -74,7 +74,10 @@ typedef struct mpm_config {
 typedef struct mpm_args {
     int32_t scheme;        /* MPM_MUSL only (args.scheme) */
     int32_t va;            /* MPM_VA_A | MPM_VA_V (args.va) */
-    int32_t constitutive;  /* MPM_LINEARELASTIC | MPM_DRUCKERPRAGER | MPM_MOHRCOULOMB */
+    int32_t constitutive;  /* MPM_LINEARELASTIC | MPM_DRUCKERPRAGER | MPM_MOHRCOULOMB
+                              (3D Mohr-Coulomb is defined by this library: a principal-space
+                              return with dilation psi; the reference's 3D mcP! is a no-op,
+                              src/materials/mohrcoulomb.jl:152-455) */
     int32_t mvl;           /* args.MVL: cell-averaged mean stress */
     double flip, pic;      /* args.FLIP, args.PIC */
     double zeta_s;         /* args.ζs, solid damping */

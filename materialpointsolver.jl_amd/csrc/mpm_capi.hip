@@ -4,6 +4,7 @@
 // the reference's Julia layouts, the per-step launch sequence, downloads in
 // the caller's particle order, HIP-event kernel timing and the triad probe.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include <cmath>
 #include <cstdio>
@@ -369,6 +370,7 @@ struct Impl : ImplBase {
             m.sinphi = R(std::sin(phi[k]));
             m.cosphi = R(std::cos(phi[k]));
             m.tanphi = R(std::tan(phi[k]));
+            m.sinpsi = R(std::sin(psi[k]));
             m.c = R(coh[k]);
             m.Hp = R(Hp[k]);
             m.cr = R(cr[k]);
@@ -464,13 +466,22 @@ struct Impl : ImplBase {
                  k_grid2<DIM, R><<<nblocks(ni), kBlock, 0, s>>>(geom, gp, R(dt));
              })) != MPM_OK)
             return rc;
+        auto g2p2 = [&](auto cm) {
+            constexpr int CM = decltype(cm)::value;
+            if (tiled)
+                k_g2p2_tiled<DIM, S, TSH, R, CM><<<sort.tiled_grid(), 64, 0, s>>>(
+                    geom, sort.tg, pp, gp, d_mats, sa, sort.tile_list, sort.seg_begin,
+                    sort.seg_end, sort.tile_count, fl);
+            else
+                k_g2p2<DIM, S, R, CM><<<pb, kBlock, 0, s>>>(geom, pp, gp, d_mats, sa, n, fl);
+        };
         if ((rc = c->launch(KID_G2P2, [&] {
-                 if (tiled)
-                     k_g2p2_tiled<DIM, S, TSH, R><<<sort.tiled_grid(), 64, 0, s>>>(
-                         geom, sort.tg, pp, gp, d_mats, sa, sort.tile_list, sort.seg_begin,
-                         sort.seg_end, sort.tile_count, fl);
+                 if (ar.constitutive == MPM_DRUCKERPRAGER)
+                     g2p2(std::integral_constant<int, 1>{});
+                 else if (ar.constitutive == MPM_MOHRCOULOMB)
+                     g2p2(std::integral_constant<int, 2>{});
                  else
-                     k_g2p2<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, d_mats, sa, n, fl);
+                     g2p2(std::integral_constant<int, 0>{});
              })) != MPM_OK)
             return rc;
         if (ar.mvl) {

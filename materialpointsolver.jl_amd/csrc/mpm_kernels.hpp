@@ -106,7 +106,8 @@ struct Material {
     R Ks, Gs, Dt, Dd;
     R kphi, qphi, qpsi, sigt, ab, tb;   // Drucker-Prager
     R eq_shear;                          // sqrt(0.333333 + 0.222222 qψ²)
-    R sinphi, cosphi, tanphi, c, Hp, cr; // Mohr-Coulomb (2D)
+    R sinphi, cosphi, tanphi, c, Hp, cr; // Mohr-Coulomb
+    R sinpsi;                            // Mohr-Coulomb (3D)
 };
 
 template <typename R>
@@ -592,8 +593,7 @@ __device__ __forceinline__ void dpP(R* s, R& eq, R& ek, const Material<R>& M) {
     }
 }
 
-// mcP! 2D plane strain (mohrcoulomb.jl:22-86).  The reference's 3D mcP! is a
-// no-op (its body is commented out, mohrcoulomb.jl:152-455).
+// mcP! 2D plane strain (mohrcoulomb.jl:22-86).  3D: mcP3 below.
 template <typename R>
 __device__ __forceinline__ void mcP2(R* s, R& eq, R& ek, const Material<R>& M) {
     R c = M.c + M.Hp * eq;
@@ -632,6 +632,161 @@ __device__ __forceinline__ void mcP2(R* s, R& eq, R& ek, const Material<R>& M) {
     eq += sqrt(R(0.666667) * (exx * exx + eyy * eyy + ezz * ezz + exy * exy * R(2)));
 }
 
+// Symmetric 3x3 eigen-decomposition of [xx,yy,zz,xy,yz,xz] by cyclic Jacobi
+// rotations (Golub & Van Loan, Matrix Computations, sec. 8.5).  lam: descending
+// eigenvalues; V[r][k]: component r of eigenvector k.
+template <typename R>
+__device__ __forceinline__ void sym3_eig(const R* s, R* lam, R (*V)[3]) {
+    R A[3][3] = {{s[0], s[3], s[5]}, {s[3], s[1], s[4]}, {s[5], s[4], s[2]}};
+    R Q[3][3] = {{R(1), R(0), R(0)}, {R(0), R(1), R(0)}, {R(0), R(0), R(1)}};
+    const R tol = sizeof(R) == 4 ? R(1e-13) : R(1e-30);
+    for (int sweep = 0; sweep < 10; ++sweep) {
+        const R off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        const R dia = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+        if (off <= tol * dia || off == R(0)) break;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+#pragma unroll
+            for (int q = p + 1; q < 3; ++q) {
+                const R apq = A[p][q];
+                if (apq == R(0)) continue;
+                const R th = (A[q][q] - A[p][p]) / (R(2) * apq);
+                const R t = (th >= R(0) ? R(1) : R(-1)) / (fabs(th) + sqrt(th * th + R(1)));
+                const R c = R(1) / sqrt(t * t + R(1)), sn = t * c;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const R akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - sn * akq;
+                    A[k][q] = sn * akp + c * akq;
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const R apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - sn * aqk;
+                    A[q][k] = sn * apk + c * aqk;
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const R qkp = Q[k][p], qkq = Q[k][q];
+                    Q[k][p] = c * qkp - sn * qkq;
+                    Q[k][q] = sn * qkp + c * qkq;
+                }
+            }
+        }
+    }
+    int o0 = 0, o1 = 1, o2 = 2;
+    const R d[3] = {A[0][0], A[1][1], A[2][2]};
+    if (d[o0] < d[o1]) { int t = o0; o0 = o1; o1 = t; }
+    if (d[o1] < d[o2]) { int t = o1; o1 = o2; o2 = t; }
+    if (d[o0] < d[o1]) { int t = o0; o0 = o1; o1 = t; }
+    lam[0] = d[o0];
+    lam[1] = d[o1];
+    lam[2] = d[o2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        V[r][0] = Q[r][o0];
+        V[r][1] = Q[r][o1];
+        V[r][2] = Q[r][o2];
+    }
+}
+
+// Principal-space Mohr-Coulomb return of p (descending), perfect plasticity
+// with dilation psi (de Souza Neto, Peric & Owen, Computational Methods for
+// Plasticity, 2008, sec. 8.2): main plane, then the edge whose ordering
+// constraint the main-plane return crosses first, then the apex c cot(phi).
+// Planes f = (s1 - s3) + (s1 + s3) sin(phi) - 2 c cos(phi); flow direction
+// (1 + sin(psi), 0, -(1 - sin(psi))).  Returns 0 when elastic.
+template <typename R>
+__device__ __forceinline__ int mc3_principal(R* p, R Dt, R Dd, R sphi, R cphi, R spsi, R c) {
+    const R k2 = R(2) * c * cphi;
+    const R f = (p[0] - p[2]) + (p[0] + p[2]) * sphi - k2;
+    if (!(f > R(0))) return 0;
+    auto dmul = [&](const R* n, R* out) {
+        out[0] = Dt * n[0] + Dd * n[1] + Dd * n[2];
+        out[1] = Dd * n[0] + Dt * n[1] + Dd * n[2];
+        out[2] = Dd * n[0] + Dd * n[1] + Dt * n[2];
+    };
+    auto dot = [](const R* a, const R* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    const R Ga[3] = {R(1) + sphi, R(0), -(R(1) - sphi)};
+    const R Na[3] = {R(1) + spsi, R(0), -(R(1) - spsi)};
+    R DNa[3], DNb[3], q[3];
+    dmul(Na, DNa);
+    const R a11 = dot(Ga, DNa);
+    const R g = f / a11;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) q[k] = p[k] - g * DNa[k];
+    if (q[0] >= q[1] && q[1] >= q[2]) {
+        p[0] = q[0];
+        p[1] = q[1];
+        p[2] = q[2];
+        return 1;
+    }
+    const bool edge12 = (R(1) - spsi) * p[0] - R(2) * p[1] + (R(1) + spsi) * p[2] < R(0);
+    R Gb[3], Nb[3], fb;
+    if (edge12) {  // s1 = s2: second plane (s2 - s3)
+        Gb[0] = R(0); Gb[1] = R(1) + sphi; Gb[2] = -(R(1) - sphi);
+        Nb[0] = R(0); Nb[1] = R(1) + spsi; Nb[2] = -(R(1) - spsi);
+        fb = (p[1] - p[2]) + (p[1] + p[2]) * sphi - k2;
+    } else {       // s2 = s3: second plane (s1 - s2)
+        Gb[0] = R(1) + sphi; Gb[1] = -(R(1) - sphi); Gb[2] = R(0);
+        Nb[0] = R(1) + spsi; Nb[1] = -(R(1) - spsi); Nb[2] = R(0);
+        fb = (p[0] - p[1]) + (p[0] + p[1]) * sphi - k2;
+    }
+    dmul(Nb, DNb);
+    const R a12 = dot(Ga, DNb), a21 = dot(Gb, DNa), a22 = dot(Gb, DNb);
+    const R det = a11 * a22 - a12 * a21;
+    if (det > R(0)) {
+        const R ga = (a22 * f - a12 * fb) / det;
+        const R gb = (a11 * fb - a21 * f) / det;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) q[k] = p[k] - ga * DNa[k] - gb * DNb[k];
+        if ((ga >= R(0) && gb >= R(0) && q[0] >= q[2]) || !(sphi > R(0))) {
+            p[0] = q[0];
+            p[1] = q[1];
+            p[2] = q[2];
+            return edge12 ? 2 : 3;
+        }
+    }
+    const R apex = c * cphi / sphi;
+    p[0] = p[1] = p[2] = apex;
+    return 4;
+}
+
+// 3D Mohr-Coulomb, defined by this library: the reference's 3D mcP! body is
+// commented out (mohrcoulomb.jl:152-455).  Conventions of its 2D mcP!
+// (mohrcoulomb.jl:22-86): cohesion softening max(c + Hp ϵq, cr), plastic
+// strain from the stress correction through the inverse elastic matrix, ϵk
+// assigned, ϵq accumulated.  s = [xx,yy,zz,xy,yz,xz].
+template <typename R>
+__device__ __forceinline__ void mcP3(R* s, R& eq, R& ek, const Material<R>& M) {
+    R c = M.c + M.Hp * eq;
+    c = (c > M.cr || c != c) ? c : M.cr;
+    R lam[3], V[3][3];
+    sym3_eig<R>(s, lam, V);
+    if (mc3_principal<R>(lam, M.Dt, M.Dd, M.sinphi, M.cosphi, M.sinpsi, c) == 0) {
+        ek = R(0);
+        return;
+    }
+    const int ri[6] = {0, 1, 2, 0, 1, 0}, ci[6] = {0, 1, 2, 1, 2, 2};
+    R d[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const R sn = lam[0] * V[ri[k]][0] * V[ci[k]][0] + lam[1] * V[ri[k]][1] * V[ci[k]][1] +
+                     lam[2] * V[ri[k]][2] * V[ci[k]][2];
+        d[k] = sn - s[k];
+        s[k] = sn;
+    }
+    const R Dt = M.Dt, Dd = M.Dd;
+    const R base = R(1) / ((Dd - Dt) * (R(2) * Dd + Dt));
+    const R exx = -(Dd * d[0] + Dt * d[0] - Dd * d[1] - Dd * d[2]) * base;
+    const R eyy = -(-Dd * d[0] + Dd * d[1] + Dt * d[1] - Dd * d[2]) * base;
+    const R ezz = -(-Dd * d[0] - Dd * d[1] + Dd * d[2] + Dt * d[2]) * base;
+    const R exy = d[3] / M.Gs, eyz = d[4] / M.Gs, exz = d[5] / M.Gs;
+    ek = exx + eyy + ezz;
+    eq += sqrt(R(0.666667) * (exx * exx + eyy * eyy + ezz * ezz +
+                              R(2) * (exy * exy + eyz * eyz + exz * exz)));
+}
+
 // Cell id of the reference's p2c (fld on x and z, cld on y), 0-based, in this
 // library's x-fastest cell order.
 template <int DIM>
@@ -645,8 +800,10 @@ __device__ __forceinline__ int cell_of(const GridGeom& g, const double* x) {
 // Everything G2P #2 does after gathering ΔF (row-major, DIM x DIM): strain
 // rate and increment, J, Ω, the constitutive update, MVL's cell sums and the
 // position commit (G2P_OS!, utils_OS.jl:1025-1153; liE!; dpP!; mcP!;
-// vollock1_OS!).  x is the step-start position.
-template <int DIM, typename R>
+// vollock1_OS!).  x is the step-start position.  CM is the constitutive
+// model (0 linear elastic, 1 Drucker-Prager, 2 Mohr-Coulomb), a template
+// parameter so that each kernel holds the registers of one model only.
+template <int DIM, typename R, int CM>
 __device__ __forceinline__ void g2p2_commit(const GridGeom& g, PartPtrs<R>& pp, GridPtrs<R>& gp,
                                             const Material<R>* __restrict__ mats,
                                             const StepArgs<R>& sa, int i, const double* x, R* F) {
@@ -699,12 +856,14 @@ __device__ __forceinline__ void g2p2_commit(const GridGeom& g, PartPtrs<R>& pp, 
 #pragma unroll
     for (int k = 0; k < NS; ++k) s[k] = pp.sig[k][i];
     liE<DIM, R>(s, F, e, M);
-    if (sa.plastic && sa.constitutive != 0) {
+    if (CM != 0 && sa.plastic) {
         R eq = pp.eq[i], ek = pp.ek[i];
-        if (sa.constitutive == 1) {
+        if constexpr (CM == 1) {
             dpP<DIM, R>(s, eq, ek, M);
         } else if constexpr (DIM == 2) {
             mcP2<R>(s, eq, ek, M);
+        } else {
+            mcP3<R>(s, eq, ek, M);
         }
         pp.eq[i] = eq;
         pp.ek[i] = ek;
@@ -726,7 +885,7 @@ __device__ __forceinline__ void g2p2_commit(const GridGeom& g, PartPtrs<R>& pp, 
 // G2P #2 + constitutive + commit (G2P_OS!, utils_OS.jl:1025-1153; liE!; dpP!;
 // mcP!; vollock1_OS!)
 // ---------------------------------------------------------------------------
-template <int DIM, int S, typename R>
+template <int DIM, int S, typename R, int CM>
 __global__ __launch_bounds__(256) void k_g2p2(GridGeom g, PartPtrs<R> pp, GridPtrs<R> gp,
                                               const Material<R>* __restrict__ mats,
                                               StepArgs<R> sa, int np, int* flags) {
@@ -761,7 +920,7 @@ __global__ __launch_bounds__(256) void k_g2p2(GridGeom g, PartPtrs<R> pp, GridPt
             }
         }
     });
-    g2p2_commit<DIM, R>(g, pp, gp, mats, sa, i, x, F);
+    g2p2_commit<DIM, R, CM>(g, pp, gp, mats, sa, i, x, F);
     raise_flag(flags, flag);
 }
 

@@ -369,7 +369,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // G2P #2 with the Δu window staged in LDS (G2P_OS!, utils_OS.jl:1025-1153),
 // then the strain / constitutive / MVL / position commit of g2p2_commit
 // ---------------------------------------------------------------------------
-template <int DIM, int S, int TSH, typename R>
+template <int DIM, int S, int TSH, typename R, int CM>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p2_tiled(
     GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, const Material<R>* __restrict__ mats,
     StepArgs<R> sa, const int* __restrict__ tile_list, const int* __restrict__ seg_begin,
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     }
                 });
             }
-            g2p2_commit<DIM, R>(g, pp, gp, mats, sa, i, x, F);
+            g2p2_commit<DIM, R, CM>(g, pp, gp, mats, sa, i, x, F);
         }
     }
     raise_flag(flags, flag);

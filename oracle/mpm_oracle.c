@@ -693,13 +693,190 @@ void oracle_dpP(oracle_model *m) {
 }
 
 /* ---------------------------------------------------------------------------
- * mcP! 2D plane strain (src/materials/mohrcoulomb.jl:22-86).  The reference's
- * 3D mcP! body is commented out (mohrcoulomb.jl:152-455), i.e. a no-op, which
- * this restatement keeps.
+ * 3D Mohr-Coulomb (build-defined).  The reference's 3D mcP! body is commented
+ * out (src/materials/mohrcoulomb.jl:152-455), so there is no reference
+ * arithmetic to restate; parity with the reference is unpinned.  This is the
+ * FP64 definition the HIP kernel is checked against, pinned in
+ * tests/test_oracle_mc3d.py by closed forms and a constrained minimisation.
+ *
+ * Same conventions as the reference's 2D mcP! (mohrcoulomb.jl:22-86): tension
+ * positive, cohesion softening c = max(c + Hp eps_q, cr), apex at c cot(phi),
+ * plastic strain increment from the stress correction through the inverse
+ * elastic matrix (Dt = K + 1.333333 G, Dd = K - 0.666667 G), eps_k assigned its
+ * trace, eps_q += sqrt(0.666667 (e_n . e_n + 2 e_s . e_s)).
+ *
+ * Return mapping in principal stresses for perfect plasticity with dilation
+ * psi (de Souza Neto, Peric & Owen, Computational Methods for Plasticity,
+ * 2008, sec. 8.2: one-vector return to the main plane, two-vector return to
+ * an edge, return to the apex), with the planes written as
+ *   f = (s1 - s3) + (s1 + s3) sin(phi) - 2 c cos(phi),  s1 >= s2 >= s3,
+ *   flow direction N = (1 + sin(psi), 0, -(1 - sin(psi))).
+ * ------------------------------------------------------------------------- */
+
+/* Eigen-decomposition of a symmetric 3x3 tensor [xx,yy,zz,xy,yz,xz] by cyclic
+ * Jacobi rotations (Golub & Van Loan, Matrix Computations, sec. 8.5).  lam:
+ * eigenvalues in descending order; V: column k = eigenvector k, row-major. */
+void oracle_sym3_eig(const double *s, double *lam, double *V) {
+    double A[3][3] = {{s[0], s[3], s[5]}, {s[3], s[1], s[4]}, {s[5], s[4], s[2]}};
+    double Q[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+    for (int sweep = 0; sweep < 10; ++sweep) {
+        double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        double dia = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+        if (off <= 1e-30 * dia || off == 0.0) break;
+        for (int p = 0; p < 2; ++p) {
+            for (int q = p + 1; q < 3; ++q) {
+                double apq = A[p][q];
+                if (apq == 0.0) continue;
+                double th = (A[q][q] - A[p][p]) / (2.0 * apq);
+                double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < 3; ++k) {
+                    double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - sn * akq;
+                    A[k][q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - sn * aqk;
+                    A[q][k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    double qkp = Q[k][p], qkq = Q[k][q];
+                    Q[k][p] = c * qkp - sn * qkq;
+                    Q[k][q] = sn * qkp + c * qkq;
+                }
+            }
+        }
+    }
+    int o[3] = {0, 1, 2};
+    double d[3] = {A[0][0], A[1][1], A[2][2]};
+    /* descending order, stable for ties */
+    if (d[o[0]] < d[o[1]]) { int t = o[0]; o[0] = o[1]; o[1] = t; }
+    if (d[o[1]] < d[o[2]]) { int t = o[1]; o[1] = o[2]; o[2] = t; }
+    if (d[o[0]] < d[o[1]]) { int t = o[0]; o[0] = o[1]; o[1] = t; }
+    for (int k = 0; k < 3; ++k) {
+        lam[k] = d[o[k]];
+        for (int r = 0; r < 3; ++r) V[r * 3 + k] = Q[r][o[k]];
+    }
+}
+
+static inline double dot3(const double *a, const double *b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* D n for the principal-space elastic matrix [[Dt,Dd,Dd],[Dd,Dt,Dd],[Dd,Dd,Dt]]. */
+static inline void dmul3(double Dt, double Dd, const double *n, double *out) {
+    out[0] = Dt * n[0] + Dd * n[1] + Dd * n[2];
+    out[1] = Dd * n[0] + Dt * n[1] + Dd * n[2];
+    out[2] = Dd * n[0] + Dd * n[1] + Dt * n[2];
+}
+
+/* Principal-space return of p (descending) in place.  Returns 0 elastic,
+ * 1 main plane, 2 edge s1 = s2, 3 edge s2 = s3, 4 apex. */
+int oracle_mc3_principal(double *p, double Dt, double Dd, double sphi, double cphi,
+                         double spsi, double c) {
+    const double k2 = 2.0 * c * cphi;
+    double f = (p[0] - p[2]) + (p[0] + p[2]) * sphi - k2;
+    if (!(f > 0.0)) return 0;
+    const double Ga[3] = {1.0 + sphi, 0.0, -(1.0 - sphi)};
+    const double Na[3] = {1.0 + spsi, 0.0, -(1.0 - spsi)};
+    double DNa[3], DNb[3];
+    dmul3(Dt, Dd, Na, DNa);
+    const double a11 = dot3(Ga, DNa);
+    double q[3];
+    const double g = f / a11;
+    for (int k = 0; k < 3; ++k) q[k] = p[k] - g * DNa[k];
+    if (q[0] >= q[1] && q[1] >= q[2]) {
+        p[0] = q[0]; p[1] = q[1]; p[2] = q[2];
+        return 1;
+    }
+    /* the edge whose ordering constraint the main-plane return crosses first */
+    const int edge12 = (1.0 - spsi) * p[0] - 2.0 * p[1] + (1.0 + spsi) * p[2] < 0.0;
+    double Gb[3], Nb[3], fb;
+    if (edge12) { /* s1 = s2: second plane (s2 - s3) */
+        Gb[0] = 0.0; Gb[1] = 1.0 + sphi; Gb[2] = -(1.0 - sphi);
+        Nb[0] = 0.0; Nb[1] = 1.0 + spsi; Nb[2] = -(1.0 - spsi);
+        fb = (p[1] - p[2]) + (p[1] + p[2]) * sphi - k2;
+    } else {      /* s2 = s3: second plane (s1 - s2) */
+        Gb[0] = 1.0 + sphi; Gb[1] = -(1.0 - sphi); Gb[2] = 0.0;
+        Nb[0] = 1.0 + spsi; Nb[1] = -(1.0 - spsi); Nb[2] = 0.0;
+        fb = (p[0] - p[1]) + (p[0] + p[1]) * sphi - k2;
+    }
+    dmul3(Dt, Dd, Nb, DNb);
+    const double a12 = dot3(Ga, DNb), a21 = dot3(Gb, DNa), a22 = dot3(Gb, DNb);
+    const double det = a11 * a22 - a12 * a21;
+    if (det > 0.0) {
+        const double ga = (a22 * f - a12 * fb) / det;
+        const double gb = (a11 * fb - a21 * f) / det;
+        for (int k = 0; k < 3; ++k) q[k] = p[k] - ga * DNa[k] - gb * DNb[k];
+        if ((ga >= 0.0 && gb >= 0.0 && q[0] >= q[2]) || !(sphi > 0.0)) {
+            p[0] = q[0]; p[1] = q[1]; p[2] = q[2];
+            return edge12 ? 2 : 3;
+        }
+    }
+    const double apex = c * cphi / sphi;
+    p[0] = p[1] = p[2] = apex;
+    return 4;
+}
+
+/* One particle: s[6] in/out ([xx,yy,zz,xy,yz,xz]), eq in/out, ek out.
+ * Returns the return type of oracle_mc3_principal. */
+int oracle_mc3_point(double *s, double *eq, double *ek, double Ks, double Gs, double phi,
+                     double psi, double coh, double Hp, double cr) {
+    const double c = jl_max(coh + Hp * (*eq), cr);
+    const double Dt = Ks + 1.333333 * Gs, Dd = Ks - 0.666667 * Gs;
+    double lam[3], V[9];
+    oracle_sym3_eig(s, lam, V);
+    int kind = oracle_mc3_principal(lam, Dt, Dd, sin(phi), cos(phi), sin(psi), c);
+    if (kind == 0) {
+        *ek = 0.0;
+        return 0;
+    }
+    double sn[6];
+    const int ri[6] = {0, 1, 2, 0, 1, 0}, ci[6] = {0, 1, 2, 1, 2, 2};
+    for (int k = 0; k < 6; ++k) {
+        const int r = ri[k], cc = ci[k];
+        sn[k] = lam[0] * V[r * 3 + 0] * V[cc * 3 + 0] + lam[1] * V[r * 3 + 1] * V[cc * 3 + 1] +
+                lam[2] * V[r * 3 + 2] * V[cc * 3 + 2];
+    }
+    double d[6];
+    for (int k = 0; k < 6; ++k) {
+        d[k] = sn[k] - s[k];
+        s[k] = sn[k];
+    }
+    const double base = 1.0 / ((Dd - Dt) * (2.0 * Dd + Dt));
+    const double exx = -(Dd * d[0] + Dt * d[0] - Dd * d[1] - Dd * d[2]) * base;
+    const double eyy = -(-Dd * d[0] + Dd * d[1] + Dt * d[1] - Dd * d[2]) * base;
+    const double ezz = -(-Dd * d[0] - Dd * d[1] + Dd * d[2] + Dt * d[2]) * base;
+    const double exy = d[3] / Gs, eyz = d[4] / Gs, exz = d[5] / Gs;
+    *ek = exx + eyy + ezz;
+    *eq += sqrt(0.666667 * (exx * exx + eyy * eyy + ezz * ezz +
+                            2.0 * (exy * exy + eyz * eyz + exz * exz)));
+    return kind;
+}
+
+/* ---------------------------------------------------------------------------
+ * mcP! 2D plane strain (src/materials/mohrcoulomb.jl:22-86); 3D: the
+ * build-defined return above.
  * ------------------------------------------------------------------------- */
 void oracle_mcP(oracle_model *m) {
-    if (m->dim == 3) return;
     const int64_t np = m->np;
+    if (m->dim == 3) {
+        #pragma omp parallel for if (par(m)) schedule(static)
+        for (int64_t ix = 0; ix < np; ++ix) {
+            int64_t k = m->nid[ix] - 1;
+            double s[6];
+            for (int j = 0; j < 6; ++j) s[j] = COL(m->sig, np, ix, j);
+            double eq = m->eps_q[ix], ek = m->eps_k[ix];
+            oracle_mc3_point(s, &eq, &ek, m->Ks[k], m->Gs[k], m->phi[k], m->psi[k], m->c[k],
+                             m->Hp[k], m->cr[k]);
+            for (int j = 0; j < 6; ++j) COL(m->sig, np, ix, j) = s[j];
+            m->eps_q[ix] = eq;
+            m->eps_k[ix] = ek;
+            deviatoric(m, ix);
+        }
+        return;
+    }
     #pragma omp parallel for if (par(m)) schedule(static)
     for (int64_t ix = 0; ix < np; ++ix) {
         int64_t k = m->nid[ix] - 1;

@@ -29,6 +29,12 @@ Pinning status of each restated function (see tests/test_oracle_*.py):
   oracle_dpP                  pinned: post-return yield residual == 0, elastic
                               states unchanged, tension apex
   oracle_mcP (2D)             pinned: post-return f <= 0, closed-form apex
+  oracle_mcP (3D) and         build-defined (the reference's 3D mcP! is a no-op):
+  oracle_sym3_eig,            pinned by eigh, f <= 0 on all six planes, the
+  oracle_mc3_*                energy-norm closest point (associative flow) found
+                              by active-set enumeration, flow-cone membership
+                              (non-associative), apex, rotation invariance;
+                              parity with the reference unpinned
   oracle_vollock              pinned: cell-mean identity
   whole step                  pinned: elastic column under gravity vs
                               σzz = -ρ g (H - z); experimental deposit profile
@@ -109,6 +115,12 @@ def load():
             getattr(lib, name).restype = None
         lib.oracle_linear_basis.argtypes = [C.c_double, C.c_double, _D, _D]
         lib.oracle_linear_basis.restype = None
+        lib.oracle_sym3_eig.argtypes = [_D, _D, _D]
+        lib.oracle_sym3_eig.restype = None
+        lib.oracle_mc3_principal.argtypes = [_D] + [C.c_double] * 6
+        lib.oracle_mc3_principal.restype = C.c_int
+        lib.oracle_mc3_point.argtypes = [_D, _D, _D] + [C.c_double] * 7
+        lib.oracle_mc3_point.restype = C.c_int
         lib.oracle_struct_size.restype = C.c_int
         if lib.oracle_struct_size() != C.sizeof(OracleModel):
             raise ImportError("oracle_model layout mismatch")
@@ -134,6 +146,31 @@ def linear_basis(d, h):
     N, dN = C.c_double(), C.c_double()
     load().oracle_linear_basis(float(d), float(h), C.byref(N), C.byref(dN))
     return N.value, dN.value
+
+
+def sym3_eig(s6):
+    """Eigenvalues (descending) and eigenvectors (columns) of [xx,yy,zz,xy,yz,xz]."""
+    s = np.ascontiguousarray(s6, dtype=np.float64)
+    lam = np.zeros(3)
+    V = np.zeros(9)
+    load().oracle_sym3_eig(s.ctypes.data_as(_D), lam.ctypes.data_as(_D), V.ctypes.data_as(_D))
+    return lam, V.reshape(3, 3)
+
+
+def mc3_principal(p3, Dt, Dd, sphi, cphi, spsi, c):
+    """Principal-space Mohr-Coulomb return; returns (p_new, kind)."""
+    p = np.ascontiguousarray(p3, dtype=np.float64).copy()
+    kind = load().oracle_mc3_principal(p.ctypes.data_as(_D), Dt, Dd, sphi, cphi, spsi, c)
+    return p, kind
+
+
+def mc3_point(s6, eq, Ks, Gs, phi, psi, c, Hp=0.0, cr=0.0):
+    """Build-defined 3D Mohr-Coulomb on one stress; returns (s, eq, ek, kind)."""
+    s = np.ascontiguousarray(s6, dtype=np.float64).copy()
+    q, k = C.c_double(eq), C.c_double(0.0)
+    kind = load().oracle_mc3_point(s.ctypes.data_as(_D), C.byref(q), C.byref(k), Ks, Gs, phi,
+                                   psi, c, Hp, cr)
+    return s, q.value, k.value, kind
 
 
 _CONST = {"linearelastic": 0, "druckerprager": 1, "mohrcoulomb": 2}

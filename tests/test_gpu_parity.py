@@ -75,6 +75,22 @@ def test_fp64_3d(mpm, orc, constitutive, va, sort):
     _compare(mpm, orc, _c3(mpm, constitutive=constitutive, va=va), 100, 1e-9, sort_interval=sort)
 
 
+@pytest.mark.parametrize("sort", [0, 5])
+@pytest.mark.parametrize("va", ["a", "v"])
+def test_fp64_3d_mohrcoulomb(mpm, orc, va, sort):
+    # 3D Mohr-Coulomb is defined by this library (principal-space return with
+    # dilation 5 deg; the reference's 3D mcP! is a no-op); every particle of
+    # this column yields within the 100 steps
+    _compare(mpm, orc, _c3(mpm, constitutive="mohrcoulomb", va=va, cohesion=200.0, psi_deg=5.0),
+             100, 1e-9, sort_interval=sort)
+
+
+def test_fp32_3d_mohrcoulomb(mpm, orc):
+    tol = dict(xi=2e-4, vs=5e-4, sigma=2e-3, eps_s=5e-4, eps_q=2e-3, omega=1e-5)
+    _compare(mpm, orc, _c3(mpm, eps="FP32", constitutive="mohrcoulomb", cohesion=200.0,
+                           psi_deg=5.0), 100, tol, sort_interval=10)
+
+
 def test_fp64_tiled_with_drift(mpm, orc):
     # sorted once at step 0 only: the block moves ~1.2 cells, so particles leave
     # their tile windows and take the direct-scatter fallback
