@@ -176,6 +176,42 @@ int mpm_count_active_nodes(mpm_ctx *ctx, int64_t *count);
  * triad. */
 int mpm_tpeak(int64_t nbytes, int reps, double *gbps);
 
+/* ---------------------------------------------------------------------------
+ * Learned return map (no reference counterpart: the reference has no
+ * surrogate).  A 16-128-128-128-7 SiLU MLP run on the matrix cores in BF16
+ * with FP32 accumulation.  Inputs per sample (16, raw units): trial stress
+ * [xx,yy,zz,xy,yz,xz] after the elastic predictor, strain increment (same
+ * order, engineering shears), accumulated plastic strain eps_q, friction angle
+ * (rad), cohesion in effect (Pa), dilation angle (rad).  Outputs (7): stress
+ * correction (returned - trial, 6) and the plastic strain increment of eps_q.
+ * Parameters: one FP32 vector of mpm_mlp_param_count() values, W1[128][16] b1
+ * W2[128][128] b2 W3[128][128] b3 W4[7][128] b4 followed by x_mean[16]
+ * x_istd[16] y_scale[7] (inputs are standardised, outputs are scaled).
+ * Samples are row-major (n x 16 inputs, n x 7 outputs), FP32.
+ * ------------------------------------------------------------------------- */
+typedef struct mpm_mlp mpm_mlp;
+
+int mpm_mlp_param_count(void);
+/* batch_capacity: largest minibatch / gradient batch (32 .. 4194304). */
+int mpm_mlp_create(int64_t batch_capacity, mpm_mlp **out);
+void mpm_mlp_destroy(mpm_mlp *m);
+const char *mpm_mlp_last_error(const mpm_mlp *m);
+int mpm_mlp_set_params(mpm_mlp *m, const float *params, int64_t n);
+int mpm_mlp_get_params(mpm_mlp *m, float *params, int64_t n);
+/* y = network(x) in output units, n rows (the fused matrix-core kernel). */
+int mpm_mlp_forward(mpm_mlp *m, const float *x, int64_t n, float *y);
+/* Mean squared error over the normalised outputs of n <= capacity samples and
+ * its gradient (first mpm_mlp_param_count() - 39 entries: the trainable ones). */
+int mpm_mlp_grad(mpm_mlp *m, const float *x, const float *y, int64_t n, float *grad,
+                 float *loss);
+/* `steps` Adam steps (lr, beta 0.9 / 0.999) on minibatches of `batch` samples
+ * drawn with replacement (seeded); loss_history (may be NULL) gets each step's
+ * minibatch loss. */
+int mpm_mlp_train(mpm_mlp *m, const float *x, const float *y, int64_t n, int64_t steps,
+                  int64_t batch, double lr, uint64_t seed, float *loss_history);
+/* Mean duration of the forward kernel on n synthetic samples (HIP events). */
+int mpm_mlp_bench(mpm_mlp *m, int64_t n, int reps, double *ms_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

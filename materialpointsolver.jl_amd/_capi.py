@@ -50,6 +50,7 @@ class MPMError(RuntimeError):
 _lib = None
 
 _D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
 _I64 = C.POINTER(C.c_int64)
 _CTX = C.c_void_p
 
@@ -71,6 +72,17 @@ _PROTOS = {
     "mpm_profile_kernel_count": (C.c_int, []),
     "mpm_count_active_nodes": (C.c_int, [_CTX, _I64]),
     "mpm_tpeak": (C.c_int, [C.c_int64, C.c_int, _D]),
+    "mpm_mlp_param_count": (C.c_int, []),
+    "mpm_mlp_create": (C.c_int, [C.c_int64, C.POINTER(_CTX)]),
+    "mpm_mlp_destroy": (None, [_CTX]),
+    "mpm_mlp_last_error": (C.c_char_p, [_CTX]),
+    "mpm_mlp_set_params": (C.c_int, [_CTX, _F, C.c_int64]),
+    "mpm_mlp_get_params": (C.c_int, [_CTX, _F, C.c_int64]),
+    "mpm_mlp_forward": (C.c_int, [_CTX, _F, C.c_int64, _F]),
+    "mpm_mlp_grad": (C.c_int, [_CTX, _F, _F, C.c_int64, _F, _F]),
+    "mpm_mlp_train": (C.c_int, [_CTX, _F, _F, C.c_int64, C.c_int64, C.c_int64, C.c_double,
+                                C.c_uint64, _F]),
+    "mpm_mlp_bench": (C.c_int, [_CTX, C.c_int64, C.c_int, _D]),
 }
 
 
@@ -104,6 +116,13 @@ def dptr(a: np.ndarray | None):
         return None
     assert a.dtype == np.float64 and (a.flags.f_contiguous or a.flags.c_contiguous)
     return a.ctypes.data_as(_D)
+
+
+def fptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_F)
 
 
 def iptr(a: np.ndarray | None):

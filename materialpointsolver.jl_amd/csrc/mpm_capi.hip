@@ -15,6 +15,7 @@
 #include "../../include/mpm_hip.h"
 #include "mpm_kernels.hpp"
 #include "mpm_sort.hpp"
+#include "mpm_surrogate.hpp"
 
 using namespace mpm;
 
