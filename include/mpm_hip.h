@@ -212,6 +212,24 @@ int mpm_mlp_train(mpm_mlp *m, const float *x, const float *y, int64_t n, int64_t
 /* Mean duration of the forward kernel on n synthetic samples (HIP events). */
 int mpm_mlp_bench(mpm_mlp *m, int64_t n, int reps, double *ms_per_launch);
 
+/* Attach a trained network as the 3D Mohr-Coulomb return of this context (FP32
+ * mode, 3D, no MVL; NULL detaches).  Each plastic step then stores the trial
+ * stress, evaluates the network for every particle, and keeps its correction
+ * where the corrected stress satisfies the yield function to
+ * |f| <= tol (2 c cos(phi) + |s1 + s3| sin(phi)) with a non-negative eps_q
+ * increment; every other yielding particle takes the exact return.  The
+ * network must outlive the attachment.  Resets the counters below. */
+int mpm_set_surrogate(mpm_ctx *ctx, mpm_mlp *m, double tol);
+/* Yielding particle-steps and fallbacks to the exact return since attaching. */
+int mpm_surrogate_stats(mpm_ctx *ctx, int64_t *plastic, int64_t *fallbacks);
+/* Log return-mapping samples of the exact 3D Mohr-Coulomb return: every
+ * stride-th particle (in the current particle order) that yields, up to
+ * `capacity` samples (0 disables and frees); samples are the network's inputs
+ * and outputs in raw units.  Resets the log. */
+int mpm_return_log(mpm_ctx *ctx, int64_t capacity, int64_t stride);
+/* Copy up to max_rows logged samples (x: rows x 16, y: rows x 7, row-major). */
+int mpm_return_log_read(mpm_ctx *ctx, float *x, float *y, int64_t max_rows, int64_t *rows);
+
 #ifdef __cplusplus
 }
 #endif

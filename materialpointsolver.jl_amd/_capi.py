@@ -83,6 +83,10 @@ _PROTOS = {
     "mpm_mlp_train": (C.c_int, [_CTX, _F, _F, C.c_int64, C.c_int64, C.c_int64, C.c_double,
                                 C.c_uint64, _F]),
     "mpm_mlp_bench": (C.c_int, [_CTX, C.c_int64, C.c_int, _D]),
+    "mpm_set_surrogate": (C.c_int, [_CTX, _CTX, C.c_double]),
+    "mpm_surrogate_stats": (C.c_int, [_CTX, _I64, _I64]),
+    "mpm_return_log": (C.c_int, [_CTX, C.c_int64, C.c_int64]),
+    "mpm_return_log_read": (C.c_int, [_CTX, _F, _F, C.c_int64, _I64]),
 }
 
 

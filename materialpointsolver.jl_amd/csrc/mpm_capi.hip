@@ -23,10 +23,12 @@ namespace {
 
 std::string g_create_error = "no error";
 
-const char* const kKernelNames[] = {"k_p2g", "k_grid1", "k_g2p1_p2g2", "k_grid2",
-                                    "k_g2p2", "k_mvl2", "k_sort", "k_permute"};
+const char* const kKernelNames[] = {"k_p2g",  "k_grid1", "k_g2p1_p2g2", "k_grid2",
+                                    "k_g2p2", "k_mvl2",  "k_sort",      "k_permute",
+                                    "k_learned_return"};
 constexpr int kNumKernels = int(sizeof(kKernelNames) / sizeof(kKernelNames[0]));
-enum { KID_P2G = 0, KID_GRID1, KID_G2P1, KID_GRID2, KID_G2P2, KID_MVL2, KID_SORT, KID_PERMUTE };
+enum { KID_P2G = 0, KID_GRID1, KID_G2P1, KID_GRID2, KID_G2P2, KID_MVL2, KID_SORT, KID_PERMUTE,
+       KID_LEARNED };
 
 constexpr int kBlock = 256;
 
@@ -118,6 +120,10 @@ struct ImplBase {
     virtual int download(mpm_ctx*, double*, double*, double*, double*, double*, double*,
                          double*, double*, double*, double*) = 0;
     virtual int count_active(mpm_ctx*, int64_t*) = 0;
+    virtual int set_surrogate(mpm_ctx*, mpm_mlp*, double tol) = 0;
+    virtual int surrogate_stats(mpm_ctx*, int64_t*, int64_t*) = 0;
+    virtual int return_log(mpm_ctx*, int64_t cap, int64_t stride) = 0;
+    virtual int return_log_read(mpm_ctx*, float*, float*, int64_t, int64_t*) = 0;
 };
 
 namespace {
@@ -168,9 +174,21 @@ struct Impl : ImplBase {
     double* d_stage = nullptr;  // np doubles, download staging
     std::vector<void*> owned;
     SortState<R> sort;
+    // learned 3D Mohr-Coulomb return (FP32 only)
+    mpm_mlp* sur = nullptr;
+    float sur_tol = 1e-3f;
+    unsigned long long* d_sur_stats = nullptr;  // [plastic, fallbacks]
+    bool have_de = false;
+    // return-mapping sample log
+    float *log_x = nullptr, *log_y = nullptr;
+    unsigned long long* d_log_count = nullptr;
+    long long log_cap = 0;
+    int log_stride = 1;
 
     ~Impl() override {
         for (void* p : owned) hipFree(p);
+        if (log_x) hipFree(log_x);
+        if (log_y) hipFree(log_y);
         sort.release();
     }
 
@@ -372,6 +390,8 @@ struct Impl : ImplBase {
             m.cosphi = R(std::cos(phi[k]));
             m.tanphi = R(std::tan(phi[k]));
             m.sinpsi = R(std::sin(psi[k]));
+            m.phi = R(phi[k]);
+            m.psi = R(psi[k]);
             m.c = R(coh[k]);
             m.Hp = R(Hp[k]);
             m.cr = R(cr[k]);
@@ -420,7 +440,7 @@ struct Impl : ImplBase {
         const mpm_config& cf = c->cfg;
         // OS_MUSL.jl:70 gravity ramp and :88 plasticity gate, evaluated in FP64
         double G = Ti < ar.Te ? ar.gravity / ar.Te * Ti : ar.gravity;
-        StepArgs<R> sa;
+        StepArgs<R> sa{};
         sa.dt = R(dt);
         sa.G = R(G);
         sa.flip = R(ar.flip);
@@ -430,6 +450,17 @@ struct Impl : ImplBase {
         sa.plastic = Ti >= ar.Te ? 1 : 0;
         sa.constitutive = ar.constitutive;
         sa.mvl = ar.mvl;
+        const bool mc3 = DIM == 3 && ar.constitutive == MPM_MOHRCOULOMB && sa.plastic;
+        sa.defer_return = (mc3 && sur) ? 1 : 0;
+        if (sa.defer_return && ar.mvl)
+            return c->fail(MPM_E_UNSUPPORTED, "mpm_step: the learned return does not combine with MVL");
+        if (mc3 && !sur && log_cap > 0) {
+            sa.log_x = log_x;
+            sa.log_y = log_y;
+            sa.log_count = d_log_count;
+            sa.log_cap = log_cap;
+            sa.log_stride = log_stride;
+        }
         const int n = int(np);
         const unsigned pb = nblocks(np), gb = nblocks(ni > nc ? ni : nc);
         hipStream_t s = c->stream;
@@ -485,6 +516,17 @@ struct Impl : ImplBase {
                      g2p2(std::integral_constant<int, 0>{});
              })) != MPM_OK)
             return rc;
+        if constexpr (std::is_same<R, float>::value) {
+            if (sa.defer_return) {
+                if ((rc = c->launch(KID_LEARNED, [&] {
+                         mpm::mlp::k_mlp_forward<SurrogateIn, SurrogateOut>
+                             <<<sur->forward_grid(np), 256, 0, s>>>(
+                                 sur->sw, sur->P, np, SurrogateIn{pp, d_mats},
+                                 SurrogateOut{pp, d_mats, sur_tol, d_sur_stats});
+                     })) != MPM_OK)
+                    return rc;
+            }
+        }
         if (ar.mvl) {
             if ((rc = c->launch(KID_MVL2, [&] {
                      k_mvl2<DIM, R><<<pb, kBlock, 0, s>>>(pp, gp, n);
@@ -522,6 +564,75 @@ struct Impl : ImplBase {
         if (e == hipSuccess) e = e2;
         if (e != hipSuccess) return c->hip_fail(e, "mpm_count_active_nodes");
         *out = (int64_t)h;
+        return MPM_OK;
+    }
+
+    int set_surrogate(mpm_ctx* c, mpm_mlp* m, double tol) override {
+        if (!std::is_same<R, float>::value && m)
+            return c->fail(MPM_E_UNSUPPORTED, "mpm_set_surrogate: the learned return runs in FP32 mode only");
+        if (m && dim != 3)
+            return c->fail(MPM_E_UNSUPPORTED, "mpm_set_surrogate: the learned return is 3D Mohr-Coulomb only");
+        if (m && !(tol > 0.0 && tol < 1.0)) return c->fail(MPM_E_ARG, "mpm_set_surrogate: tol outside (0, 1)");
+        int rc;
+        if (m && !have_de) {
+            for (int k = 0; k < 6; ++k)
+                if ((rc = alloc(c, &pp.de[k], np, "strain increment")) != MPM_OK) return rc;
+            have_de = true;
+        }
+        if (!d_sur_stats && (rc = alloc(c, &d_sur_stats, 2, "surrogate stats")) != MPM_OK) return rc;
+        hipError_t e = hipMemsetAsync(d_sur_stats, 0, 2 * sizeof(unsigned long long), c->stream);
+        if (e == hipSuccess && m) e = hipStreamSynchronize(m->stream);  // weights ready
+        if (e != hipSuccess) return c->hip_fail(e, "mpm_set_surrogate");
+        sur = m;
+        sur_tol = float(tol);
+        return MPM_OK;
+    }
+
+    int surrogate_stats(mpm_ctx* c, int64_t* plastic, int64_t* fallbacks) override {
+        unsigned long long h[2] = {0, 0};
+        if (d_sur_stats) {
+            hipError_t e = hipMemcpyAsync(h, d_sur_stats, sizeof h, hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) return c->hip_fail(e, "mpm_surrogate_stats");
+        }
+        if (plastic) *plastic = int64_t(h[0]);
+        if (fallbacks) *fallbacks = int64_t(h[1]);
+        return MPM_OK;
+    }
+
+    int return_log(mpm_ctx* c, int64_t cap, int64_t stride) override {
+        if (cap < 0 || stride < 1) return c->fail(MPM_E_ARG, "mpm_return_log: cap >= 0, stride >= 1");
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (log_x) hipFree(log_x);
+        if (log_y) hipFree(log_y);
+        log_x = log_y = nullptr;
+        log_cap = 0;
+        if (e == hipSuccess && cap > 0) {
+            e = hipMalloc((void**)&log_x, sizeof(float) * 16 * cap);
+            if (e == hipSuccess) e = hipMalloc((void**)&log_y, sizeof(float) * 7 * cap);
+        }
+        int rc;
+        if (!d_log_count && (rc = alloc(c, &d_log_count, 1, "log counter")) != MPM_OK) return rc;
+        if (e == hipSuccess) e = hipMemset(d_log_count, 0, sizeof(unsigned long long));
+        if (e != hipSuccess) return c->hip_fail(e, "mpm_return_log");
+        log_cap = cap;
+        log_stride = int(stride);
+        return MPM_OK;
+    }
+
+    int return_log_read(mpm_ctx* c, float* x, float* y, int64_t max_rows, int64_t* rows) override {
+        unsigned long long cnt = 0;
+        hipError_t e = hipSuccess;
+        if (d_log_count) {
+            e = hipMemcpyAsync(&cnt, d_log_count, sizeof cnt, hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        }
+        long long n = (long long)cnt < log_cap ? (long long)cnt : log_cap;
+        if (n > max_rows) n = max_rows;
+        if (e == hipSuccess && n > 0 && x) e = hipMemcpy(x, log_x, sizeof(float) * 16 * n, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && n > 0 && y) e = hipMemcpy(y, log_y, sizeof(float) * 7 * n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return c->hip_fail(e, "mpm_return_log_read");
+        if (rows) *rows = n;
         return MPM_OK;
     }
 
@@ -786,6 +897,26 @@ const char* mpm_profile_kernel_name(int k) {
 }
 
 int mpm_profile_kernel_count(void) { return kNumKernels; }
+
+int mpm_set_surrogate(mpm_ctx* c, mpm_mlp* m, double tol) {
+    if (!c) return MPM_E_ARG;
+    return c->impl->set_surrogate(c, m, tol);
+}
+
+int mpm_surrogate_stats(mpm_ctx* c, int64_t* plastic, int64_t* fallbacks) {
+    if (!c) return MPM_E_ARG;
+    return c->impl->surrogate_stats(c, plastic, fallbacks);
+}
+
+int mpm_return_log(mpm_ctx* c, int64_t capacity, int64_t stride) {
+    if (!c) return MPM_E_ARG;
+    return c->impl->return_log(c, capacity, stride);
+}
+
+int mpm_return_log_read(mpm_ctx* c, float* x, float* y, int64_t max_rows, int64_t* rows) {
+    if (!c || max_rows < 0) return MPM_E_ARG;
+    return c->impl->return_log_read(c, x, y, max_rows, rows);
+}
 
 int mpm_count_active_nodes(mpm_ctx* c, int64_t* count) {
     if (!c || !count) return MPM_E_ARG;

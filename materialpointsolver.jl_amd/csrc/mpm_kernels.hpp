@@ -81,6 +81,7 @@ struct PartPtrs {
     int* nid;         // 0-based material id
     int* pid;         // original particle index (for downloads after sorting)
     int* p2c;         // step-start cell id (MVL only)
+    R* de[6];         // strain increment of the step (deferred 3D Mohr-Coulomb return only)
 };
 
 template <typename R>
@@ -108,6 +109,7 @@ struct Material {
     R eq_shear;                          // sqrt(0.333333 + 0.222222 qψ²)
     R sinphi, cosphi, tanphi, c, Hp, cr; // Mohr-Coulomb
     R sinpsi;                            // Mohr-Coulomb (3D)
+    R phi, psi;                          // angles (rad), inputs of the learned return map
 };
 
 template <typename R>
@@ -117,6 +119,16 @@ struct StepArgs {
     int plastic;       // Ti >= Te
     int constitutive;  // 0 LE, 1 DP, 2 MC
     int mvl;
+    // 3D Mohr-Coulomb only: defer_return leaves the return mapping to the
+    // learned-return kernel (trial stress in sig, strain increment in de);
+    // log_* (when log_x is set) records return-mapping samples of every
+    // log_stride-th particle that yields: 16 inputs and 7 outputs each.
+    int defer_return;
+    int log_stride;
+    float* log_x;
+    float* log_y;
+    unsigned long long* log_count;
+    long long log_cap;
 };
 
 // ---------------------------------------------------------------------------
@@ -758,14 +770,36 @@ __device__ __forceinline__ int mc3_principal(R* p, R Dt, R Dd, R sphi, R cphi, R
 // strain from the stress correction through the inverse elastic matrix, ϵk
 // assigned, ϵq accumulated.  s = [xx,yy,zz,xy,yz,xz].
 template <typename R>
-__device__ __forceinline__ void mcP3(R* s, R& eq, R& ek, const Material<R>& M) {
-    R c = M.c + M.Hp * eq;
-    c = (c > M.cr || c != c) ? c : M.cr;
+__device__ __forceinline__ R mc_cohesion(const Material<R>& M, R eq) {
+    const R c = M.c + M.Hp * eq;
+    return (c > M.cr || c != c) ? c : M.cr;  // max(c + Hp ϵq, cr)
+}
+
+// Plastic strain bookkeeping of a stress correction d = s_new - s_trial
+// (inverse elastic matrix; ϵk assigned, ϵq accumulated).
+template <typename R>
+__device__ __forceinline__ void mc_strain(const R* d, R& eq, R& ek, const Material<R>& M) {
+    const R Dt = M.Dt, Dd = M.Dd;
+    const R base = R(1) / ((Dd - Dt) * (R(2) * Dd + Dt));
+    const R exx = -(Dd * d[0] + Dt * d[0] - Dd * d[1] - Dd * d[2]) * base;
+    const R eyy = -(-Dd * d[0] + Dd * d[1] + Dt * d[1] - Dd * d[2]) * base;
+    const R ezz = -(-Dd * d[0] - Dd * d[1] + Dd * d[2] + Dt * d[2]) * base;
+    const R exy = d[3] / M.Gs, eyz = d[4] / M.Gs, exz = d[5] / M.Gs;
+    ek = exx + eyy + ezz;
+    eq += sqrt(R(0.666667) * (exx * exx + eyy * eyy + ezz * ezz +
+                              R(2) * (exy * exy + eyz * eyz + exz * exz)));
+}
+
+// Returns the kind of return (0 elastic, 1 main plane, 2/3 edge, 4 apex).
+template <typename R>
+__device__ __forceinline__ int mcP3(R* s, R& eq, R& ek, const Material<R>& M) {
+    const R c = mc_cohesion<R>(M, eq);
     R lam[3], V[3][3];
     sym3_eig<R>(s, lam, V);
-    if (mc3_principal<R>(lam, M.Dt, M.Dd, M.sinphi, M.cosphi, M.sinpsi, c) == 0) {
+    const int kind = mc3_principal<R>(lam, M.Dt, M.Dd, M.sinphi, M.cosphi, M.sinpsi, c);
+    if (kind == 0) {
         ek = R(0);
-        return;
+        return 0;
     }
     const int ri[6] = {0, 1, 2, 0, 1, 0}, ci[6] = {0, 1, 2, 1, 2, 2};
     R d[6];
@@ -776,15 +810,30 @@ __device__ __forceinline__ void mcP3(R* s, R& eq, R& ek, const Material<R>& M) {
         d[k] = sn - s[k];
         s[k] = sn;
     }
-    const R Dt = M.Dt, Dd = M.Dd;
-    const R base = R(1) / ((Dd - Dt) * (R(2) * Dd + Dt));
-    const R exx = -(Dd * d[0] + Dt * d[0] - Dd * d[1] - Dd * d[2]) * base;
-    const R eyy = -(-Dd * d[0] + Dd * d[1] + Dt * d[1] - Dd * d[2]) * base;
-    const R ezz = -(-Dd * d[0] - Dd * d[1] + Dd * d[2] + Dt * d[2]) * base;
-    const R exy = d[3] / M.Gs, eyz = d[4] / M.Gs, exz = d[5] / M.Gs;
-    ek = exx + eyy + ezz;
-    eq += sqrt(R(0.666667) * (exx * exx + eyy * eyy + ezz * ezz +
-                              R(2) * (exy * exy + eyz * eyz + exz * exz)));
+    mc_strain<R>(d, eq, ek, M);
+    return kind;
+}
+
+// Appends one return-mapping sample (raw units): inputs trial stress, strain
+// increment, ϵq, ϕ, cohesion in effect, ψ; outputs stress correction, Δϵq.
+template <typename R>
+__device__ __forceinline__ void log_return(const StepArgs<R>& sa, const R* s0, const R* e, R eq0,
+                                           const R* s1, R eq1, const Material<R>& M) {
+    const unsigned long long slot = atomicAdd(sa.log_count, 1ull);
+    if ((long long)slot >= sa.log_cap) return;
+    float* x = sa.log_x + slot * 16;
+    float* y = sa.log_y + slot * 7;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        x[k] = float(s0[k]);
+        x[6 + k] = float(e[k]);
+        y[k] = float(s1[k] - s0[k]);
+    }
+    x[12] = float(eq0);
+    x[13] = float(M.phi);
+    x[14] = float(mc_cohesion<R>(M, eq0));
+    x[15] = float(M.psi);
+    y[6] = float(eq1 - eq0);
 }
 
 // Cell id of the reference's p2c (fld on x and z, cld on y), 0-based, in this
@@ -863,10 +912,24 @@ __device__ __forceinline__ void g2p2_commit(const GridGeom& g, PartPtrs<R>& pp, 
         } else if constexpr (DIM == 2) {
             mcP2<R>(s, eq, ek, M);
         } else {
-            mcP3<R>(s, eq, ek, M);
+            if (sa.defer_return) {
+                // trial stress is stored below; the learned-return kernel finishes
+#pragma unroll
+                for (int k = 0; k < 6; ++k) pp.de[k][i] = e[k];
+            } else if (sa.log_x && i % sa.log_stride == 0) {
+                R s0[6];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) s0[k] = s[k];
+                const R eq0 = eq;
+                if (mcP3<R>(s, eq, ek, M) != 0) log_return<R>(sa, s0, e, eq0, s, eq, M);
+            } else {
+                mcP3<R>(s, eq, ek, M);
+            }
         }
-        pp.eq[i] = eq;
-        pp.ek[i] = ek;
+        if (!(DIM == 3 && CM == 2) || !sa.defer_return) {
+            pp.eq[i] = eq;
+            pp.ek[i] = ek;
+        }
     }
 #pragma unroll
     for (int k = 0; k < NS; ++k) pp.sig[k][i] = s[k];

@@ -9,7 +9,95 @@
 #include <vector>
 
 #include "../../include/mpm_hip.h"
+#include "mpm_kernels.hpp"
 #include "mpm_mlp.hpp"
+
+namespace mpm {
+
+// Network inputs of particle p after a deferred 3D Mohr-Coulomb step: trial
+// stress, strain increment, ϵq, ϕ, cohesion in effect, ψ.
+struct SurrogateIn {
+    PartPtrs<float> pp;
+    const Material<float>* mats;
+    __device__ void load(long long p, int half, float* f) const {
+        if (half == 0) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) f[k] = pp.sig[k][p];
+            f[6] = pp.de[0][p];
+            f[7] = pp.de[1][p];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) f[k] = pp.de[2 + k][p];
+            const Material<float>& M = mats[pp.nid[p]];
+            const float eq = pp.eq[p];
+            f[4] = eq;
+            f[5] = M.phi;
+            f[6] = mc_cohesion<float>(M, eq);
+            f[7] = M.psi;
+        }
+    }
+};
+
+// The learned return of particle p with the yield-function residual check:
+// a yielding trial stress takes the network's correction (and its ϵq
+// increment) when the corrected stress lies on the Mohr-Coulomb surface to
+// |f| <= tol * (2 c cos ϕ + |σ1 + σ3| sin ϕ) and the ϵq increment is
+// non-negative; otherwise the exact return (mcP3).  Elastic trial stresses
+// stay as they are (ϵk = 0, as mcP3).
+// stats[0] counts yielding particles, stats[1] fallbacks to the exact return.
+struct SurrogateOut {
+    PartPtrs<float> pp;
+    const Material<float>* mats;
+    float tol;
+    unsigned long long* stats;
+    __device__ void store(long long p, const float* y) const {
+        float s[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s[k] = pp.sig[k][p];
+        const Material<float> M = mats[pp.nid[p]];
+        float eq = pp.eq[p], ek = 0.0f;
+        const float c = mc_cohesion<float>(M, eq);
+        float lam[3], V[3][3];
+        sym3_eig<float>(s, lam, V);
+        const float f = (lam[0] - lam[2]) + (lam[0] + lam[2]) * M.sinphi - 2.0f * c * M.cosphi;
+        const bool plastic = f > 0.0f;
+        bool fallback = false;
+        if (plastic) {
+            float sp[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sp[k] = s[k] + y[k];
+            float lp[3], Vp[3][3];
+            sym3_eig<float>(sp, lp, Vp);
+            const float F = (lp[0] - lp[2]) + (lp[0] + lp[2]) * M.sinphi - 2.0f * c * M.cosphi;
+            float scale = 2.0f * c * M.cosphi + fabsf(lp[0] + lp[2]) * M.sinphi;
+            scale = fmaxf(scale, 1e-6f * (fabsf(lp[0]) + fabsf(lp[2])) + 1e-30f);
+            const bool ok = fabsf(F) <= tol * scale && y[6] >= 0.0f && isfinite(F) &&
+                            isfinite(y[6]);
+            if (ok) {
+                float eq_unused = 0.0f;
+                mc_strain<float>(y, eq_unused, ek, M);  // ϵk from the stress correction
+                eq += y[6];                             // ϵq increment from the network
+#pragma unroll
+                for (int k = 0; k < 6; ++k) pp.sig[k][p] = sp[k];
+            } else {
+                fallback = true;
+                mcP3<float>(s, eq, ek, M);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) pp.sig[k][p] = s[k];
+            }
+            pp.eq[p] = eq;
+        }
+        pp.ek[p] = ek;
+        const unsigned long long np_ = __popcll(__ballot(plastic));
+        const unsigned long long nf_ = __popcll(__ballot(fallback));
+        if ((threadIdx.x & 63) == 0) {
+            if (np_) atomicAdd(&stats[0], np_);
+            if (nf_) atomicAdd(&stats[1], nf_);
+        }
+    }
+};
+
+}  // namespace mpm
 
 struct mpm_mlp {
     std::string error;

@@ -144,6 +144,36 @@ class HIPModel:
         self._check(self.lib.mpm_count_active_nodes(self.ctx, capi.iptr(out)))
         return int(out[0])
 
+    # -- learned 3D Mohr-Coulomb return ----------------------------------------
+    def return_log(self, capacity, stride=1):
+        """Start logging exact 3D Mohr-Coulomb return-mapping samples."""
+        self._check(self.lib.mpm_return_log(self.ctx, int(capacity), int(stride)))
+
+    def read_return_log(self, max_rows=None):
+        """Logged samples as (X (n, 16), Y (n, 7)) float32 arrays."""
+        cap = int(max_rows) if max_rows is not None else 1 << 26
+        rows = np.zeros(1, dtype=np.int64)
+        self._check(self.lib.mpm_return_log_read(self.ctx, None, None, cap, capi.iptr(rows)))
+        n = int(rows[0])
+        X = np.zeros((n, 16), dtype=np.float32)
+        Y = np.zeros((n, 7), dtype=np.float32)
+        if n:
+            self._check(self.lib.mpm_return_log_read(self.ctx, capi.fptr(X), capi.fptr(Y), n,
+                                                     capi.iptr(rows)))
+        return X, Y
+
+    def set_surrogate(self, surrogate, tol=1e-3):
+        """Use a trained Surrogate as the 3D Mohr-Coulomb return (None detaches)."""
+        h = surrogate.h if surrogate is not None else None
+        self._check(self.lib.mpm_set_surrogate(self.ctx, h, float(tol)))
+        self._surrogate = surrogate  # keep the network alive while attached
+
+    def surrogate_stats(self):
+        """(yielding particle-steps, fallbacks to the exact return) since attaching."""
+        out = np.zeros(2, dtype=np.int64)
+        self._check(self.lib.mpm_surrogate_stats(self.ctx, capi.iptr(out[:1]), capi.iptr(out[1:])))
+        return int(out[0]), int(out[1])
+
     def profile_enable(self, on=True):
         self._check(self.lib.mpm_profile_enable(self.ctx, 1 if on else 0))
 

@@ -127,3 +127,84 @@ def test_training_learns_the_return_map(mpm, orc):
         assert err1 < 0.35 and err1 < 0.4 * err0
     finally:
         S.close()
+
+
+def _mc_column(mpm, h, eps="FP32", nsteps=400):
+    return mpm.scenes.column_collapse_3d(h=h, width=0.16, height=0.08, box_xy=0.4, box_z=0.12,
+                                         nsteps=nsteps, eps=eps, constitutive="mohrcoulomb",
+                                         cohesion=200.0, psi_deg=0.0)
+
+
+def test_return_log_matches_exact_return(mpm, orc):
+    args, grid, mp, attr, bc = _mc_column(mpm, 0.02)
+    m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=10)
+    try:
+        m.return_log(200000, 1)
+        m.run(m.step_args(args), float(args.ΔT), 0.0, 60)
+        X, Y = m.read_return_log()
+    finally:
+        m.close()
+    print(f"logged {len(X)} samples", flush=True)
+    assert len(X) > 1000
+    K, G = float(attr.Ks[0]), float(attr.Gs[0])
+    rng = np.random.default_rng(0)
+    worst = 0.0
+    for r in rng.choice(len(X), 300, replace=False):
+        x = X[r].astype(np.float64)
+        out, eq1, ek, kind = orc.mc3_point(x[:6], x[12], K, G, x[13], x[15], x[14])
+        assert kind != 0
+        scale = np.abs(x[:6]).max() + x[14]
+        worst = max(worst, np.abs(Y[r, :6] - (out - x[:6])).max() / scale)
+        assert abs(Y[r, 6] - (eq1 - x[12])) <= 2e-3 * abs(eq1 - x[12]) + 1e-9
+    print(f"worst stress-correction difference / |σ|: {worst:.2e}", flush=True)
+    assert worst <= 1e-4
+
+
+def test_learned_return_in_the_step(mpm):
+    # plain run with logging -> training -> run with the learned return
+    nsteps = 300
+    args, grid, mp, attr, bc = _mc_column(mpm, 0.01, nsteps=nsteps)
+    sa_dt = float(args.ΔT)
+    x0 = np.asarray(mp.ξ, dtype=np.float64)
+    plain = mpm.HIPModel(grid, mp, attr, bc, sort_interval=10)
+    try:
+        plain.return_log(2_000_000, 1)
+        plain.run(plain.step_args(args), sa_dt, 0.0, nsteps)
+        X, Y = plain.read_return_log()
+        ref = plain.download()
+    finally:
+        plain.close()
+    print(f"{mp.np} particles, {len(X)} return samples", flush=True)
+    S = mpm.Surrogate(batch_capacity=4096, seed=0)
+    U = mpm.Surrogate(batch_capacity=64, seed=0)
+    try:
+        S.fit_normalisation(X, Y)
+        U.set_params(S.params())  # same normalisation, untrained weights
+        hist = S.train(X, Y, steps=3000, batch=4096, lr=2e-3, seed=5)
+        print(f"training loss {hist[:20].mean():.4f} -> {hist[-20:].mean():.5f}", flush=True)
+        results = {}
+        for name, net in (("trained", S), ("untrained", U)):
+            m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=10)
+            try:
+                m.set_surrogate(net, tol=1e-3)
+                m.run(m.step_args(args), sa_dt, 0.0, nsteps)
+                got = m.download()
+                plastic, fb = m.surrogate_stats()
+            finally:
+                m.close()
+            run_ref = mpm.scenes.runout(ref["xi"], x0)
+            run_got = mpm.scenes.runout(got["xi"], x0)
+            d = _rel(got["xi"] - x0, ref["xi"] - x0)
+            results[name] = (plastic, fb, run_got / run_ref - 1.0, d)
+            print(f"{name}: yielding particle-steps {plastic}, fallbacks {fb} "
+                  f"({fb / max(plastic, 1):.3f}), runout vs exact {run_got / run_ref - 1:+.2e}, "
+                  f"displacement rel L2 {d:.2e}", flush=True)
+    finally:
+        S.close()
+        U.close()
+    tr, un = results["trained"], results["untrained"]
+    assert tr[0] > 0 and un[0] > 0
+    # what the trained network gains: it serves yielding particles the untrained one cannot
+    assert tr[1] / tr[0] < un[1] / un[0]
+    # and the run stays on the exact solution at the runout tolerance of FP32 mode
+    assert abs(tr[2]) <= 5e-3
