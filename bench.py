@@ -52,6 +52,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs (FETCH_SIZE, WRITE_SIZE)")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--ens-instances", type=int, default=256)
+    ap.add_argument("--ens-steps", type=int, default=200)
+    ap.add_argument("--ens-seq-steps", type=int, default=40)
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-return leg")
+    ap.add_argument("--sur-steps", type=int, default=100)
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -258,7 +264,159 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if not a.no_ensemble:
+        out["batched"] = ensemble_leg(a, mpm)
+    if not a.no_surrogate:
+        out["surrogate"] = surrogate_leg(a, mpm)
     print(json.dumps(out), flush=True)
+
+
+def ensemble_leg(a, mpm):
+    """Batched ensemble of independent 2D column collapses packed into one grid
+    (one launch per kernel for all instances, graph-captured steps) against the
+    same instances solved one after another, each on its own grid."""
+    import numpy as np
+
+    t = time.time()
+    args, grid, mp, attr, bc, lay = mpm.scenes.ensemble_2d(n_instances=a.ens_instances)
+    dt = float(args.ΔT)
+    ninst = len(lay["ranges"])
+    print(f"# ensemble: {ninst} instances, {mp.np} particles, {grid.ni} nodes "
+          f"(setup {time.time() - t:.1f}s)", file=sys.stderr, flush=True)
+    x0 = np.asarray(mp.ξ, dtype=np.float64)
+    m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=10)
+    sa = m.step_args(args)
+    res = {}
+    Ti = m.run(sa, dt, 0.0, 20, graph=True)
+    for graph in (False, True):
+        m.sync()
+        t0 = time.perf_counter()
+        Ti = m.run(sa, dt, Ti, a.ens_steps, graph=graph)
+        m.sync()
+        el = time.perf_counter() - t0
+        res["graph" if graph else "eager"] = ninst * a.ens_steps / el
+    st = m.download()
+    m.close()
+    steps_done = 20 + 2 * a.ens_steps
+    run = []
+    for k, (lo, hi) in enumerate(lay["ranges"]):
+        ox = lay["origin"][k][0]
+        run.append(mpm.scenes.runout(st["xi"][lo:hi], x0[lo:hi]) - ox)
+    run = np.array(run)
+    seq = {}
+    for graph in (False, True):
+        tot = 0.0
+        for k in range(ninst):
+            sc = mpm.scenes.ensemble_instance(lay, k)
+            s = mpm.HIPModel(sc[1], sc[2], sc[3], sc[4], sort_interval=10)
+            ss = s.step_args(sc[0])
+            T = s.run(ss, dt, 0.0, 10, graph=graph)
+            s.sync()
+            t0 = time.perf_counter()
+            s.run(ss, dt, T, a.ens_seq_steps, graph=graph)
+            s.sync()
+            tot += time.perf_counter() - t0
+            s.close()
+        seq["graph" if graph else "eager"] = ninst * a.ens_seq_steps / tot
+    best_seq = max(seq.values())
+    out = {
+        "metric": "instance-steps/s",
+        "workload": f"{ninst} independent 2D DP column collapses (aspect U(0.5,4), 5k-60k particles, "
+                    f"phi U(20,40) deg, cohesion lognormal median 1 kPa), {mp.np} particles, FP32",
+        "batched_graph": res["graph"],
+        "batched_eager": res["eager"],
+        "sequential_eager": seq["eager"],
+        "sequential_graph": seq["graph"],
+        "speedup_vs_best_sequential": res["graph"] / best_seq,
+        "particle_steps_per_s": res["graph"] * mp.np / ninst,
+        "runout_after_steps": steps_done,
+        "runout_m": {"min": float(run.min()), "median": float(np.median(run)),
+                     "max": float(run.max())},
+    }
+    print(f"# ensemble: batched {res['graph']:.0f} (graph) / {res['eager']:.0f} (eager) "
+          f"instance-steps/s; sequential {seq['eager']:.0f} / {seq['graph']:.0f}", file=sys.stderr,
+          flush=True)
+    return out
+
+
+def surrogate_leg(a, mpm):
+    """The learned 3D Mohr-Coulomb return against the exact return on the 3D
+    column: samples logged from a plain run, network trained on the GPU, then
+    both forms timed from the same state; fallback rate and runout difference."""
+    import numpy as np
+
+    sc = mpm.scenes.column_collapse_3d(h=a.h, eps="FP32", constitutive="mohrcoulomb",
+                                       phi_deg=30.0, cohesion=1.0e3)
+    args, grid, mp, attr, bc = sc
+    dt = float(args.ΔT)
+    x0 = np.asarray(mp.ξ, dtype=np.float64)
+    n = int(mp.np)
+    plain = mpm.HIPModel(grid, mp, attr, bc, sort_interval=a.sort_interval)
+    sa = plain.step_args(args)
+    plain.return_log(2_000_000, 16)
+    t0 = time.perf_counter()
+    plain.run(sa, dt, 0.0, a.sur_steps)
+    plain.sync()
+    X, Y = plain.read_return_log()
+    plain.return_log(0, 1)
+    S = mpm.Surrogate(batch_capacity=4096, seed=0)
+    S.fit_normalisation(X, Y)
+    t1 = time.perf_counter()
+    hist = S.train(X, Y, steps=2000, batch=4096, lr=2e-3, seed=1)
+    t_train = time.perf_counter() - t1
+    print(f"# surrogate: {len(X)} samples, trained 2000 steps in {t_train:.1f}s, loss "
+          f"{hist[:20].mean():.4f} -> {hist[-20:].mean():.5f}", file=sys.stderr, flush=True)
+    # timed: each form runs 2 x sur_steps from the initial state; the second half is timed
+    plain.close()
+    times, results = {}, {}
+    for mode in ("exact", "learned"):
+        m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=a.sort_interval)
+        if mode == "learned":
+            m.set_surrogate(S, tol=1e-3)
+        T = m.run(sa, dt, 0.0, a.sur_steps)
+        m.sync()
+        m.profile_enable(True)
+        t0 = time.perf_counter()
+        m.run(sa, dt, T, a.sur_steps)
+        m.sync()
+        times[mode] = time.perf_counter() - t0
+        prof = m.profile_read()
+        m.profile_enable(False)
+        st = m.download()
+        results[mode] = (st, prof, m.surrogate_stats() if mode == "learned" else (0, 0))
+        m.close()
+    S.close()
+    st_e, st_l = results["exact"][0], results["learned"][0]
+    r_e = mpm.scenes.runout(st_e["xi"], x0)
+    r_l = mpm.scenes.runout(st_l["xi"], x0)
+    plastic, fb = results["learned"][2]
+    ms_k, cnt_k = results["learned"][1].get("k_learned_return", (0.0, 0))
+    flops = mpm.surrogate.Surrogate.FLOPS_PER_SAMPLE * n
+    tflops = flops / (ms_k * 1e-3 / cnt_k) / 1e12 if cnt_k else None
+    out = {
+        "workload": f"3D Mohr-Coulomb column (phi 30 deg, c 1 kPa), {n} particles, FP32, "
+                    f"{a.sur_steps} timed steps after {a.sur_steps}",
+        "exact_particle_steps_per_s": n * a.sur_steps / times["exact"],
+        "learned_particle_steps_per_s": n * a.sur_steps / times["learned"],
+        "speed_ratio_learned_over_exact": times["exact"] / times["learned"],
+        "yielding_particle_steps": plastic,
+        "fallback_rate": fb / plastic if plastic else None,
+        "residual_tol": 1e-3,
+        "runout_rel_diff": r_l / r_e - 1.0,
+        "displacement_rel_l2": float(np.linalg.norm(st_l["xi"] - st_e["xi"]) /
+                                     np.linalg.norm(st_e["xi"] - x0)),
+        "samples": int(len(X)),
+        "train_seconds": t_train,
+        "roofline": {"kernel": "k_learned_return", "bound": "mfma",
+                     "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": tflops / 2500.0 if tflops else None,
+                     "flops_per_launch": flops},
+    }
+    print(f"# surrogate: exact {out['exact_particle_steps_per_s']:.3e}, learned "
+          f"{out['learned_particle_steps_per_s']:.3e} particle-steps/s, fallback "
+          f"{out['fallback_rate']}, runout diff {out['runout_rel_diff']:+.2e}, MLP kernel "
+          f"{tflops} TFLOP/s", file=sys.stderr, flush=True)
+    return out
 
 
 def cpu_baseline(a, args, grid, mp, attr, bc):

@@ -133,6 +133,14 @@ int mpm_step(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti);
 int mpm_run(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int64_t nsteps,
             double *Ti_out);
 
+/* mpm_run with the steps between two sorts captured into HIP graphs and
+ * replayed: runs of two or more steps whose kernel arguments do not change
+ * (Ti >= Te) become one graph launch; sorts and gravity-ramp steps stay eager.
+ * Graphs are cached per context (run length, arguments, dt, buffer parity).
+ * Same results as mpm_run up to the order of the atomic sums. */
+int mpm_run_graph(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int64_t nsteps,
+                  double *Ti_out);
+
 /* Wait for all work on the context's stream; reports range/finite flags the
  * kernels raised.  Replaces KAsync(getBackend(Val(:ROCm))) (src/solver.jl:128). */
 int mpm_sync(mpm_ctx *ctx);

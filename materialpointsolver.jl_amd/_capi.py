@@ -62,6 +62,7 @@ _PROTOS = {
     "mpm_upload_bc": (C.c_int, [_CTX, _I64, _D, _I64, _D, _I64, _D]),
     "mpm_step": (C.c_int, [_CTX, C.POINTER(MPMArgs), C.c_double, C.c_double]),
     "mpm_run": (C.c_int, [_CTX, C.POINTER(MPMArgs), C.c_double, C.c_double, C.c_int64, _D]),
+    "mpm_run_graph": (C.c_int, [_CTX, C.POINTER(MPMArgs), C.c_double, C.c_double, C.c_int64, _D]),
     "mpm_sync": (C.c_int, [_CTX]),
     "mpm_download_particles": (C.c_int, [_CTX, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
     "mpm_destroy": (None, [_CTX]),

@@ -64,6 +64,20 @@ struct mpm_ctx {
     bool use_tiles = true;  // LDS-binned scatters when the particles are tile-sorted
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> pool;
+    // captured step graphs of mpm_run_graph, keyed by what their kernel
+    // arguments depend on
+    struct Graph {
+        long long len;
+        int parity, tiled;
+        mpm_args args;
+        double dt;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
+    void drop_graphs() {
+        for (auto& g : graphs) hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -117,6 +131,8 @@ struct ImplBase {
     virtual int upload_bc(mpm_ctx*, const int64_t*, const double*, const int64_t*, const double*,
                           const int64_t*, const double*) = 0;
     virtual int step(mpm_ctx*, const mpm_args& a, double dt, double Ti) = 0;
+    virtual int run_graph(mpm_ctx*, const mpm_args& a, double dt, double Ti, int64_t nsteps,
+                          double* Ti_out) = 0;
     virtual int download(mpm_ctx*, double*, double*, double*, double*, double*, double*,
                          double*, double*, double*, double*) = 0;
     virtual int count_active(mpm_ctx*, int64_t*) = 0;
@@ -281,14 +297,17 @@ struct Impl : ImplBase {
         if ((rc = sort.init(c->cfg, geom, np)) != MPM_OK)
             return c->fail(rc, "sort buffers: " + sort.error);
         if (sort.enabled) {
-            char* sR = nullptr;
-            if ((rc = alloc(c, &sR, sizeof(R) * np, "sort scratch")) != MPM_OK) return rc;
-            sort.scratchR = sR;
-            char* sx = nullptr;
-            if ((rc = alloc(c, &sx, (sizeof(R) == 4 ? 8 : 4) * np, "sort scratch")) != MPM_OK)
-                return rc;
-            if (sizeof(R) == 4) sort.scratch8 = sx;
-            else sort.scratch4 = sx;
+            hipError_t ea = hipSuccess;
+            SortState<R>::for_each_stream(pp, dim, [&](auto*& ptr) {
+                using T = std::remove_pointer_t<std::remove_reference_t<decltype(ptr)>>;
+                T* q = nullptr;
+                if (ea == hipSuccess) ea = dalloc(&q, np);
+                if (ea == hipSuccess) {
+                    owned.push_back((void*)q);
+                    sort.alt.push_back((void*)q);
+                }
+            });
+            if (ea != hipSuccess) return c->fail(MPM_E_NOMEM, std::string("sort buffers: ") + hipGetErrorString(ea));
         }
         return MPM_OK;
     }
@@ -466,10 +485,7 @@ struct Impl : ImplBase {
         hipStream_t s = c->stream;
         int* fl = c->d_flags;
         int rc;
-        if (cf.sort_interval > 0 && (c->resort || (c->steps % cf.sort_interval) == 0)) {
-            if ((rc = sort.run(c, pp, s)) != MPM_OK) return rc;
-            c->resort = false;
-        }
+        (void)cf;
         const bool tiled = sort.enabled && sort.tiles_valid && c->use_tiles;
         constexpr int TSH = DIM == 3 ? 2 : 3;
         if ((rc = c->launch(KID_P2G, [&] {
@@ -533,14 +549,88 @@ struct Impl : ImplBase {
                  })) != MPM_OK)
                 return rc;
         }
-        c->steps += 1;
+        return MPM_OK;
+    }
+
+    // the kernels of one step (no sort)
+    int kernels(mpm_ctx* c, const mpm_args& a, double dt, double Ti) {
+        const bool lin = c->cfg.basis == MPM_LINEAR;
+        if (dim == 2) return lin ? step_impl<2, 2>(c, a, dt, Ti) : step_impl<2, 3>(c, a, dt, Ti);
+        return lin ? step_impl<3, 2>(c, a, dt, Ti) : step_impl<3, 3>(c, a, dt, Ti);
+    }
+
+    // the periodic cell sort, when due before step number c->steps
+    int sort_if_due(mpm_ctx* c) {
+        const int K = c->cfg.sort_interval;
+        if (K > 0 && (c->resort || (c->steps % K) == 0)) {
+            int rc = sort.run(c, pp, c->stream);
+            if (rc != MPM_OK) return rc;
+            c->resort = false;
+        }
         return MPM_OK;
     }
 
     int step(mpm_ctx* c, const mpm_args& a, double dt, double Ti) override {
-        const bool lin = c->cfg.basis == MPM_LINEAR;
-        if (dim == 2) return lin ? step_impl<2, 2>(c, a, dt, Ti) : step_impl<2, 3>(c, a, dt, Ti);
-        return lin ? step_impl<3, 2>(c, a, dt, Ti) : step_impl<3, 3>(c, a, dt, Ti);
+        int rc = sort_if_due(c);
+        if (rc == MPM_OK) rc = kernels(c, a, dt, Ti);
+        if (rc == MPM_OK) c->steps += 1;
+        return rc;
+    }
+
+    // Runs of steps between two sorts whose kernel arguments do not change
+    // (Ti >= Te: full gravity, plasticity on) are captured once into a HIP
+    // graph and replayed; sorts, ramp steps and runs of one step are eager.
+    // The graph's pointers are those of the sort parity it was captured at.
+    int run_graph(mpm_ctx* c, const mpm_args& a, double dt, double Ti, int64_t nsteps,
+                  double* Ti_out) override {
+        const int K = c->cfg.sort_interval;
+        int64_t k = 0;
+        int rc = MPM_OK;
+        while (k < nsteps && rc == MPM_OK) {
+            if ((rc = sort_if_due(c)) != MPM_OK) break;
+            long long L = nsteps - k;
+            if (K > 0) L = std::min<long long>(L, K - (c->steps % K));
+            if (Ti < a.Te || L < 2 || c->prof_on) {
+                if ((rc = kernels(c, a, dt, Ti)) != MPM_OK) break;
+                c->steps += 1;
+                Ti += dt;
+                k += 1;
+                continue;
+            }
+            const int tiled = (sort.enabled && sort.tiles_valid && c->use_tiles) ? 1 : 0;
+            hipGraphExec_t exec = nullptr;
+            for (auto& g : c->graphs)
+                if (g.len == L && g.parity == sort.parity && g.tiled == tiled && g.dt == dt &&
+                    std::memcmp(&g.args, &a, sizeof a) == 0)
+                    exec = g.exec;
+            if (!exec) {
+                if (c->graphs.size() >= 8) c->drop_graphs();
+                hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+                if (e != hipSuccess) return c->hip_fail(e, "hipStreamBeginCapture");
+                double Tc = Ti;
+                for (long long j = 0; j < L && rc == MPM_OK; ++j) {
+                    rc = kernels(c, a, dt, Tc);
+                    Tc += dt;
+                }
+                hipGraph_t graph = nullptr;
+                e = hipStreamEndCapture(c->stream, &graph);
+                if (rc != MPM_OK) {
+                    if (graph) hipGraphDestroy(graph);
+                    break;
+                }
+                if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                if (graph) hipGraphDestroy(graph);
+                if (e != hipSuccess) return c->hip_fail(e, "step graph capture");
+                c->graphs.push_back({L, sort.parity, tiled, a, dt, exec});
+            }
+            hipError_t e = hipGraphLaunch(exec, c->stream);
+            if (e != hipSuccess) return c->hip_fail(e, "hipGraphLaunch");
+            for (long long j = 0; j < L; ++j) Ti += dt;
+            c->steps += L;
+            k += L;
+        }
+        if (Ti_out) *Ti_out = Ti;
+        return rc;
     }
 
     template <int DIM, int S>
@@ -585,6 +675,7 @@ struct Impl : ImplBase {
         if (e != hipSuccess) return c->hip_fail(e, "mpm_set_surrogate");
         sur = m;
         sur_tol = float(tol);
+        c->drop_graphs();
         return MPM_OK;
     }
 
@@ -617,6 +708,7 @@ struct Impl : ImplBase {
         if (e != hipSuccess) return c->hip_fail(e, "mpm_return_log");
         log_cap = cap;
         log_stride = int(stride);
+        c->drop_graphs();
         return MPM_OK;
     }
 
@@ -791,8 +883,7 @@ int mpm_upload_bc(mpm_ctx* c, const int64_t* vx_idx, const double* vx_val, const
     return rc;
 }
 
-int mpm_step(mpm_ctx* c, const mpm_args* a, double dt, double Ti) {
-    if (!c) return MPM_E_ARG;
+static int mpm_step_check(mpm_ctx* c, const mpm_args* a, double dt) {
     if (!a) return c->fail(MPM_E_ARG, "mpm_step: NULL args");
     if (!c->have_particles || !c->have_materials || !c->have_bc)
         return c->fail(MPM_E_STATE, "mpm_step: upload particles, materials and bc first");
@@ -801,6 +892,13 @@ int mpm_step(mpm_ctx* c, const mpm_args* a, double dt, double Ti) {
     if (a->va != MPM_VA_A && a->va != MPM_VA_V) return c->fail(MPM_E_ARG, "mpm_step: va must be :a or :v");
     if (a->constitutive < 0 || a->constitutive > 2)
         return c->fail(MPM_E_UNSUPPORTED, "mpm_step: constitutive must be linearelastic, druckerprager or mohrcoulomb");
+    return MPM_OK;
+}
+
+int mpm_step(mpm_ctx* c, const mpm_args* a, double dt, double Ti) {
+    if (!c) return MPM_E_ARG;
+    int rc = mpm_step_check(c, a, dt);
+    if (rc != MPM_OK) return rc;
     return c->impl->step(c, *a, dt, Ti);
 }
 
@@ -814,6 +912,20 @@ int mpm_run(mpm_ctx* c, const mpm_args* a, double dt, double Ti, int64_t nsteps,
     }
     if (Ti_out) *Ti_out = Ti;
     return MPM_OK;
+}
+
+int mpm_run_graph(mpm_ctx* c, const mpm_args* a, double dt, double Ti, int64_t nsteps,
+                  double* Ti_out) {
+    if (!c) return MPM_E_ARG;
+    if (nsteps < 0) return c->fail(MPM_E_ARG, "mpm_run_graph: nsteps < 0");
+    if (nsteps == 0) {
+        if (Ti_out) *Ti_out = Ti;
+        return MPM_OK;
+    }
+    // the same argument checks as mpm_step, once
+    int rc = mpm_step_check(c, a, dt);
+    if (rc != MPM_OK) return rc;
+    return c->impl->run_graph(c, *a, dt, Ti, nsteps, Ti_out);
 }
 
 int mpm_sync(mpm_ctx* c) {
@@ -844,6 +956,7 @@ int mpm_download_particles(mpm_ctx* c, double* xi, double* vs, double* sigma, do
 void mpm_destroy(mpm_ctx* c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
+    c->drop_graphs();
     delete c->impl;
     for (auto& r : c->recs) {
         hipEventDestroy(r.e0);

@@ -13,15 +13,18 @@
 // have pairwise distinct bases, which is what the tiled scatter kernels need
 // (mpm_tiled.hpp).  The key sort is the rocPRIM radix sort (through hipCUB)
 // over (key, index) pairs; the permutation is one gather kernel per particle
-// stream into a scratch stream of the same element size, followed by a
-// pointer swap.  The original particle index travels with the particles (pid)
-// so downloads return the caller's order.
+// stream into that stream's own alternate buffer, followed by a pointer swap,
+// so every stream flips between the same two buffers and the particle
+// pointers take one of two configurations (parity), which is what lets
+// captured step graphs be reused across sorts.  The original particle index
+// travels with the particles (pid) so downloads return the caller's order.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "../../include/mpm_hip.h"
 #include "mpm_kernels.hpp"
@@ -97,11 +100,11 @@ struct SortState {
     int *idx = nullptr, *idx_alt = nullptr;
     void* temp = nullptr;
     size_t temp_bytes = 0;
-    // Scratch streams swap roles with particle streams on every permutation, so
-    // the caller allocates and frees them together with the particle streams.
-    void* scratchR = nullptr;   // np * sizeof(R)
-    void* scratch8 = nullptr;   // np * 8 (only when R is 4 bytes)
-    void* scratch4 = nullptr;   // np * 4 (int streams, only when R is 8 bytes)
+    // Alternate buffers, one per permuted stream in for_each_stream order;
+    // they swap roles with the particle streams on every permutation, so the
+    // caller allocates and frees them together with the particle streams.
+    std::vector<void*> alt;
+    int parity = 0;  // number of permutations, mod 2
 
     // Sorted keys of the current order (tile | rank | base), read by the tiled kernels.
     const unsigned* sorted_keys() const { return keys_alt; }
@@ -169,15 +172,34 @@ struct SortState {
     }
 
     template <typename T>
-    hipError_t permute(T*& arr, void*& scratch, const int* order, hipStream_t s) {
+    hipError_t permute(T*& arr, void*& other, const int* order, hipStream_t s) {
         const unsigned nb = unsigned((np + 255) / 256);
-        k_gather<T><<<nb, 256, 0, s>>>(arr, order, (T*)scratch, int(np));
+        k_gather<T><<<nb, 256, 0, s>>>(arr, order, (T*)other, int(np));
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         T* old = arr;
-        arr = (T*)scratch;
-        scratch = (void*)old;
+        arr = (T*)other;
+        other = (void*)old;
         return hipSuccess;
+    }
+
+    // Every stream the sort permutes, in a fixed order: f(pointer, element size).
+    template <typename F>
+    static void for_each_stream(PartPtrs<R>& pp, int dim, F&& f) {
+        const int ns = dim == 2 ? 4 : 6;
+        for (int a = 0; a < dim; ++a) f(pp.x[a]);
+        f(pp.J);
+        for (int a = 0; a < dim; ++a) f(pp.v[a]);
+        for (int k = 0; k < ns; ++k) f(pp.sig[k]);
+        for (int k = 0; k < ns; ++k) f(pp.eps[k]);
+        f(pp.ms);
+        f(pp.vol);
+        f(pp.vol0);
+        f(pp.eq);
+        f(pp.ek);
+        f(pp.ev);
+        f(pp.nid);
+        f(pp.pid);
     }
 
     // Sort and permute every particle stream, then rebuild the tile segments.
@@ -202,28 +224,17 @@ struct SortState {
         });
         if (rc != MPM_OK) return rc;
         const int* order = idx_alt;
-        const int ns = dim == 2 ? 4 : 6;
         hipError_t e = hipSuccess;
         rc = c->launch(7, [&] {
-            void*& s8 = sizeof(R) == 8 ? scratchR : scratch8;
-            void*& sR = scratchR;
-            void*& s4 = sizeof(R) == 4 ? scratchR : scratch4;
-            for (int a = 0; a < dim && e == hipSuccess; ++a) e = permute(pp.x[a], s8, order, s);
-            if (e == hipSuccess) e = permute(pp.J, s8, order, s);
-            for (int a = 0; a < dim && e == hipSuccess; ++a) e = permute(pp.v[a], sR, order, s);
-            for (int k = 0; k < ns && e == hipSuccess; ++k) e = permute(pp.sig[k], sR, order, s);
-            for (int k = 0; k < ns && e == hipSuccess; ++k) e = permute(pp.eps[k], sR, order, s);
-            if (e == hipSuccess) e = permute(pp.ms, sR, order, s);
-            if (e == hipSuccess) e = permute(pp.vol, sR, order, s);
-            if (e == hipSuccess) e = permute(pp.vol0, sR, order, s);
-            if (e == hipSuccess) e = permute(pp.eq, sR, order, s);
-            if (e == hipSuccess) e = permute(pp.ek, sR, order, s);
-            if (e == hipSuccess) e = permute(pp.ev, sR, order, s);
-            if (e == hipSuccess) e = permute(pp.nid, s4, order, s);
-            if (e == hipSuccess) e = permute(pp.pid, s4, order, s);
+            size_t k = 0;
+            for_each_stream(pp, dim, [&](auto*& ptr) {
+                if (e == hipSuccess) e = permute(ptr, alt[k], order, s);
+                ++k;
+            });
         });
         if (rc != MPM_OK) return rc;
         if (e != hipSuccess) return c->hip_fail(e, "k_permute");
+        parity ^= 1;
         // tile segments of the new order and the ordered list of non-empty tiles
         rc = c->launch(6, [&] {
             const unsigned tb = unsigned((tg.ntiles + 255) / 256);

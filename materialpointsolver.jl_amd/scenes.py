@@ -125,3 +125,127 @@ def runout(xi: np.ndarray, xi0: np.ndarray, axis=0, threshold=None) -> float:
 
 def kinetic_energy(ms: np.ndarray, vs: np.ndarray) -> float:
     return float(0.5 * np.sum(ms * np.sum(vs * vs, axis=1)))
+
+
+def ensemble_2d(*, n_instances=256, seed=2024, h=0.0025, np_range=(5_000, 60_000),
+                aspect_range=(0.5, 4.0), phi_deg_range=(20.0, 40.0), cohesion_median=1.0e3,
+                cohesion_sigma_ln=0.5, runout_factor=3.0, gap_cells=4, row_width=6.0,
+                dt=2e-5, nsteps=20_000, eps="FP32", va="a", device="HIP"):
+    """A parameter-uncertainty ensemble of independent plane-strain column
+    collapses packed side by side into one background grid.
+
+    Instance k (seeded): column aspect ratio H/W ~ U(aspect_range), particle
+    count ~ U(np_range) at 2x2 particles per cell, Drucker-Prager friction
+    ~ U(phi_deg_range), cohesion ~ LogNormal(ln cohesion_median,
+    cohesion_sigma_ln), its own material id.  Its box has a no-slip floor, a
+    slip wall behind the column and a slip wall runout_factor * H + W ahead
+    of it.  Boxes are placed in rows of at most row_width metres with
+    gap_cells empty cells between them, so no uGIMP stencil (one node beyond
+    a particle's cell) reaches two instances and the instances advance
+    independently in one set of kernel launches.
+
+    Returns (args, grid, mp, attr, bc, layout); layout holds per-instance
+    particle ranges (caller order), box origins and lengths, and parameters.
+    """
+    rng = np.random.default_rng(seed)
+    lp = h / 2
+    inst = []
+    for k in range(n_instances):
+        a = rng.uniform(*aspect_range)
+        n = rng.uniform(*np_range)
+        cells = n / 4.0
+        wc = max(2, int(round(math.sqrt(cells / a))))
+        hc = max(2, int(round(wc * a)))
+        Lc = wc + int(math.ceil(runout_factor * hc))
+        inst.append(dict(wc=wc, hc=hc, Lc=Lc, phi=math.radians(rng.uniform(*phi_deg_range)),
+                         c=float(math.exp(rng.normal(math.log(cohesion_median), cohesion_sigma_ln)))))
+    # shelf packing in rows of at most row_width (cells), tallest first per row
+    order = sorted(range(n_instances), key=lambda k: -inst[k]["hc"])
+    maxw = int(row_width / h)
+    x, y, row_h = gap_cells, gap_cells, 0
+    for k in order:
+        b = inst[k]
+        if x + b["Lc"] + gap_cells > maxw and x > gap_cells:
+            y += row_h + 2 + gap_cells
+            x, row_h = gap_cells, 0
+        b["cx"], b["cy"] = x, y
+        x += b["Lc"] + gap_cells
+        row_h = max(row_h, b["hc"])
+    ncx = maxw
+    ncy = y + row_h + 2 + gap_cells
+    grid = UserGrid2D(ϵ=eps, x1=0.0, x2=ncx * h, y1=0.0, y2=ncy * h, dx=h, dy=h, NIC=9)
+    pts, nid, ranges = [], [], []
+    start = 0
+    for k, b in enumerate(inst):
+        x0, y0 = b["cx"] * h, b["cy"] * h
+        px = _lattice(x0, x0 + b["wc"] * h, lp)
+        py = _lattice(y0, y0 + b["hc"] * h, lp)
+        p = meshbuilder(px, py)
+        pts.append(p)
+        nid.append(np.full(len(p), k + 1, dtype=np.int64))
+        ranges.append((start, start + len(p)))
+        start += len(p)
+    pts = np.concatenate(pts)
+    mp = UserParticle2D(ϵ=eps, NIC=9, dx=lp, dy=lp, ξ=pts, ρs=np.full(len(pts), RHO))
+    nm = n_instances
+    attr = UserProperty(ϵ=eps, nid=np.concatenate(nid), ν=[NU] * nm, Es=[E] * nm, Gs=[GS] * nm,
+                        Ks=[KS] * nm, σt=[0.0] * nm, ϕ=[b["phi"] for b in inst], ϕr=[0.0] * nm,
+                        ψ=[0.0] * nm, c=[b["c"] for b in inst], cr=[0.0] * nm, Hp=[0.0] * nm)
+    # boundary conditions by node coordinate, instance by instance, on the
+    # (column, row) view of the reference node order (rows from the top)
+    nnx, nny = grid.nnx, grid.nny
+    vx_idx = np.zeros(grid.ni, dtype=np.int64)
+    vy_idx = np.zeros(grid.ni, dtype=np.int64)
+    vx2 = vx_idx.reshape(nnx, nny)
+    vy2 = vy_idx.reshape(nnx, nny)
+    for b in inst:
+        xa, xb = b["cx"], b["cx"] + b["Lc"]
+        ya, yb = b["cy"], b["cy"] + b["hc"] + 2
+        cols = np.arange(max(xa - 2, 0), min(xb + 2, nnx - 1) + 1)
+        iys = np.arange(max(ya - 2, 0), min(yb, nny - 1) + 1)
+        rows = nny - 1 - iys
+        gxs, gys = np.meshgrid(cols, iys, indexing="ij")
+        floor = gys <= ya
+        walls = (gxs <= xa) | (gxs >= xb)
+        sub = np.ix_(cols, rows)
+        vx2[sub] |= (floor | walls).astype(np.int64)
+        vy2[sub] |= floor.astype(np.int64)
+    bc = UserVBoundary2D(ϵ=eps, vx_s_idx=vx_idx, vx_s_val=np.zeros(grid.ni),
+                         vy_s_idx=vy_idx, vy_s_val=np.zeros(grid.ni))
+    args = UserArgs2D(Ttol=dt * nsteps, Te=0.0, ΔT=dt, FLIP=1, PIC=0,
+                      constitutive="druckerprager", basis="uGIMP", MVL=False, device=device,
+                      coupling="OS", scheme="MUSL", va=va, gravity=-9.8, ζs=0.0,
+                      project_name="ensemble2d", project_path=".", ϵ=eps, progressbar=False)
+    layout = dict(h=h, ranges=ranges,
+                  origin=[(b["cx"] * h, b["cy"] * h) for b in inst],
+                  width=[b["wc"] * h for b in inst], height=[b["hc"] * h for b in inst],
+                  length=[b["Lc"] * h for b in inst], phi=[b["phi"] for b in inst],
+                  cohesion=[b["c"] for b in inst])
+    return args, grid, mp, attr, bc, layout
+
+
+def ensemble_instance(layout, k, *, eps="FP32", dt=2e-5, nsteps=20_000, device="HIP", pad=2):
+    """Instance k of an ensemble_2d layout as a standalone scene on its own
+    tight grid (same particles, material and walls): the sequential baseline."""
+    h = layout["h"]
+    lp = h / 2
+    W, H, L = layout["width"][k], layout["height"][k], layout["length"][k]
+    grid = UserGrid2D(ϵ=eps, x1=-pad * h, x2=L + pad * h, y1=-pad * h, y2=H + 2 * h + pad * h,
+                      dx=h, dy=h, NIC=9)
+    pts = meshbuilder(_lattice(0.0, W, lp), _lattice(0.0, H, lp))
+    mp = UserParticle2D(ϵ=eps, NIC=9, dx=lp, dy=lp, ξ=pts, ρs=np.full(len(pts), RHO))
+    attr = UserProperty(ϵ=eps, nid=np.ones(mp.np), ν=[NU], Es=[E], Gs=[GS], Ks=[KS], σt=[0.0],
+                        ϕ=[layout["phi"][k]], ϕr=[0.0], ψ=[0.0], c=[layout["cohesion"][k]],
+                        cr=[0.0], Hp=[0.0])
+    gx = np.rint(grid.ξ[:, 0] / h).astype(np.int64)
+    gy = np.rint(grid.ξ[:, 1] / h).astype(np.int64)
+    Lc = int(round(L / h))
+    vx_idx = ((gx <= 0) | (gx >= Lc) | (gy <= 0)).astype(np.int64)
+    vy_idx = (gy <= 0).astype(np.int64)
+    bc = UserVBoundary2D(ϵ=eps, vx_s_idx=vx_idx, vx_s_val=np.zeros(grid.ni),
+                         vy_s_idx=vy_idx, vy_s_val=np.zeros(grid.ni))
+    args = UserArgs2D(Ttol=dt * nsteps, Te=0.0, ΔT=dt, FLIP=1, PIC=0,
+                      constitutive="druckerprager", basis="uGIMP", MVL=False, device=device,
+                      coupling="OS", scheme="MUSL", va="a", gravity=-9.8, ζs=0.0,
+                      project_name="instance", project_path=".", ϵ=eps, progressbar=False)
+    return args, grid, mp, attr, bc

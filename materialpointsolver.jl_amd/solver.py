@@ -114,10 +114,13 @@ class HIPModel:
     def step(self, sargs, dt, Ti):
         self._check(self.lib.mpm_step(self.ctx, capi.C.byref(sargs), float(dt), float(Ti)))
 
-    def run(self, sargs, dt, Ti, nsteps):
+    def run(self, sargs, dt, Ti, nsteps, graph=False):
+        """nsteps steps from Ti with the submit_work! clock; returns the final Ti.
+        graph=True replays captured HIP graphs of the steps between sorts."""
         out = capi.C.c_double(0.0)
-        self._check(self.lib.mpm_run(self.ctx, capi.C.byref(sargs), float(dt), float(Ti),
-                                     int(nsteps), capi.C.byref(out)))
+        fn = self.lib.mpm_run_graph if graph else self.lib.mpm_run
+        self._check(fn(self.ctx, capi.C.byref(sargs), float(dt), float(Ti), int(nsteps),
+                       capi.C.byref(out)))
         return out.value
 
     def sync(self):
