@@ -83,6 +83,10 @@ typedef struct mpm_args {
     double zeta_s;         /* args.ζs, solid damping */
     double gravity;        /* args.gravity, on the last axis */
     double Te;             /* args.Te: gravity ramp end and plasticity gate */
+    double gravity_x;      /* body force on the first axis (no reference counterpart: the
+                              reference's gravity acts on the last axis only; 0 reproduces it).
+                              For slope-aligned frames; ramped like gravity. */
+    double gravity_y;      /* 3D: body force on the second axis (ignored in 2D); 0 by default */
 } mpm_args;
 
 typedef struct mpm_ctx mpm_ctx;

@@ -36,7 +36,8 @@ class MPMConfig(C.Structure):
 class MPMArgs(C.Structure):
     _fields_ = [("scheme", C.c_int32), ("va", C.c_int32), ("constitutive", C.c_int32),
                 ("mvl", C.c_int32), ("flip", C.c_double), ("pic", C.c_double),
-                ("zeta_s", C.c_double), ("gravity", C.c_double), ("Te", C.c_double)]
+                ("zeta_s", C.c_double), ("gravity", C.c_double), ("Te", C.c_double),
+                ("gravity_x", C.c_double), ("gravity_y", C.c_double)]
 
 
 class MPMError(RuntimeError):

@@ -459,6 +459,12 @@ struct Impl : ImplBase {
         const mpm_config& cf = c->cfg;
         // OS_MUSL.jl:70 gravity ramp and :88 plasticity gate, evaluated in FP64
         double G = Ti < ar.Te ? ar.gravity / ar.Te * Ti : ar.gravity;
+        Gravity<R> Gv{};
+        const double gx = Ti < ar.Te ? ar.gravity_x / ar.Te * Ti : ar.gravity_x;
+        const double gy = Ti < ar.Te ? ar.gravity_y / ar.Te * Ti : ar.gravity_y;
+        Gv.g[0] = R(gx);
+        Gv.g[1] = R(DIM == 3 ? gy : G);
+        Gv.g[2] = R(DIM == 3 ? G : 0.0);
         StepArgs<R> sa{};
         sa.dt = R(dt);
         sa.G = R(G);
@@ -491,10 +497,10 @@ struct Impl : ImplBase {
         if ((rc = c->launch(KID_P2G, [&] {
                  if (tiled)
                      k_p2g_tiled<DIM, S, TSH, R><<<sort.tiled_grid(), 64, 0, s>>>(
-                         geom, sort.tg, pp, gp, R(G), sort.sorted_keys(), sort.tile_list,
+                         geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
                          sort.seg_begin, sort.seg_end, sort.tile_count, fl);
                  else
-                     k_p2g<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, R(G), n, fl);
+                     k_p2g<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, Gv, n, fl);
              })) != MPM_OK)
             return rc;
         if ((rc = c->launch(KID_GRID1, [&] {

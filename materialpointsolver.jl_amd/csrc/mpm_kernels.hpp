@@ -112,6 +112,14 @@ struct Material {
     R phi, psi;                          // angles (rad), inputs of the learned return map
 };
 
+// Body force per unit mass on each axis.  The reference applies gravity on
+// the last axis only (utils_OS.jl:445-448,490-494); the other components are
+// an extension for slope-aligned frames and are zero by default.
+template <typename R>
+struct Gravity {
+    R g[3];
+};
+
 template <typename R>
 struct StepArgs {
     R dt, G, flip, pic, zeta;
@@ -318,7 +326,7 @@ __device__ __forceinline__ void load_x(const double* const* xs, int i, double* x
 // ---------------------------------------------------------------------------
 template <int DIM, int S, typename R>
 __global__ __launch_bounds__(256) void k_p2g(GridGeom g, PartPtrs<R> pp, GridPtrs<R> gp,
-                                             R G, int np, int* flags) {
+                                             Gravity<R> G, int np, int* flags) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     double x[3];
@@ -346,13 +354,13 @@ __global__ __launch_bounds__(256) void k_p2g(GridGeom g, PartPtrs<R> pp, GridPtr
             atomic_add(&gp.p[0][n], N * pv[0]);
             atomic_add(&gp.p[1][n], N * pv[1]);
             if constexpr (DIM == 2) {
-                atomic_add(&gp.f[0][n], -vol * (dNx * s0 + dNy * s3));
-                atomic_add(&gp.f[1][n], -vol * (dNy * s1 + dNx * s3) + NiM * G);
+                atomic_add(&gp.f[0][n], -vol * (dNx * s0 + dNy * s3) + NiM * G.g[0]);
+                atomic_add(&gp.f[1][n], -vol * (dNy * s1 + dNx * s3) + NiM * G.g[1]);
             } else {
                 atomic_add(&gp.p[2][n], N * pv[2]);
-                atomic_add(&gp.f[0][n], -vol * (dNx * s0 + dNy * s3 + dNz * s5));
-                atomic_add(&gp.f[1][n], -vol * (dNy * s1 + dNx * s3 + dNz * s4));
-                atomic_add(&gp.f[2][n], -vol * (dNz * s2 + dNx * s5 + dNy * s4) + NiM * G);
+                atomic_add(&gp.f[0][n], -vol * (dNx * s0 + dNy * s3 + dNz * s5) + NiM * G.g[0]);
+                atomic_add(&gp.f[1][n], -vol * (dNy * s1 + dNx * s3 + dNz * s4) + NiM * G.g[1]);
+                atomic_add(&gp.f[2][n], -vol * (dNz * s2 + dNx * s5 + dNy * s4) + NiM * G.g[2]);
             }
         }
     });

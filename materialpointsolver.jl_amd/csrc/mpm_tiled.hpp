@@ -139,19 +139,19 @@ __device__ __forceinline__ int rank_round(const unsigned* __restrict__ keys, int
 // (utils_OS.jl:466-494).
 template <int DIM, typename R>
 __device__ __forceinline__ void p2g_values(R N, R dNx, R dNy, R dNz, R m, R vol, const R* pv,
-                                           const R* sg, R G, R* c) {
+                                           const R* sg, const Gravity<R>& G, R* c) {
     const R NiM = m * N;
     c[0] = NiM;
     c[1] = N * pv[0];
     c[2] = N * pv[1];
     if constexpr (DIM == 2) {
-        c[3] = -vol * (dNx * sg[0] + dNy * sg[3]);
-        c[4] = -vol * (dNy * sg[1] + dNx * sg[3]) + NiM * G;
+        c[3] = -vol * (dNx * sg[0] + dNy * sg[3]) + NiM * G.g[0];
+        c[4] = -vol * (dNy * sg[1] + dNx * sg[3]) + NiM * G.g[1];
     } else {
         c[3] = N * pv[2];
-        c[4] = -vol * (dNx * sg[0] + dNy * sg[3] + dNz * sg[5]);
-        c[5] = -vol * (dNy * sg[1] + dNx * sg[3] + dNz * sg[4]);
-        c[6] = -vol * (dNz * sg[2] + dNx * sg[5] + dNy * sg[4]) + NiM * G;
+        c[4] = -vol * (dNx * sg[0] + dNy * sg[3] + dNz * sg[5]) + NiM * G.g[0];
+        c[5] = -vol * (dNy * sg[1] + dNx * sg[3] + dNz * sg[4]) + NiM * G.g[1];
+        c[6] = -vol * (dNz * sg[2] + dNx * sg[5] + dNy * sg[4]) + NiM * G.g[2];
     }
 }
 
@@ -160,7 +160,7 @@ __device__ __forceinline__ void p2g_values(R N, R dNx, R dNy, R dNz, R m, R vol,
 // ---------------------------------------------------------------------------
 template <int DIM, int S, int TSH, typename R>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_p2g_tiled(
-    GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, R G,
+    GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, Gravity<R> G,
     const unsigned* __restrict__ keys, const int* __restrict__ tile_list,
     const int* __restrict__ seg_begin, const int* __restrict__ seg_end,
     const int* __restrict__ count_ptr, int* flags) {

@@ -44,7 +44,11 @@ end
 struct MPMArgs
     scheme::Int32; va::Int32; constitutive::Int32; mvl::Int32
     flip::Float64; pic::Float64; zeta_s::Float64; gravity::Float64; Te::Float64
+    gravity_x::Float64; gravity_y::Float64   # extension; 0 = the package's gravity
 end
+
+# body force on the other axes, read from the user-extension slot when present
+extra_gravity(ext, name) = hasproperty(ext, name) ? Float64(getproperty(ext, name)) : 0.0
 
 const CONSTITUTIVE = Dict(:linearelastic => Int32(0), :druckerprager => Int32(1),
                           :mohrcoulomb => Int32(2))
@@ -123,7 +127,8 @@ function procedure!(
 ) where {T1, T2, va}
     a = MPMArgs(Int32(0), va == :a ? Int32(0) : Int32(1), CONSTITUTIVE[args.constitutive],
         Int32(args.MVL), Float64(args.FLIP), Float64(args.PIC), Float64(args.ζs),
-        Float64(args.gravity), Float64(args.Te))
+        Float64(args.gravity), Float64(args.Te), extra_gravity(args.ext, :gravity_x),
+        extra_gravity(args.ext, :gravity_y))
     p = grid.ctx.ptr
     check(p, ccall((:mpm_step, LIB), Cint, (Ptr{Cvoid}, Ref{MPMArgs}, Cdouble, Cdouble),
         p, a, Float64(ΔT), Float64(Ti)))

@@ -249,3 +249,126 @@ def ensemble_instance(layout, k, *, eps="FP32", dt=2e-5, nsteps=20_000, device="
                       coupling="OS", scheme="MUSL", va="a", gravity=-9.8, ζs=0.0,
                       project_name="instance", project_path=".", ϵ=eps, progressbar=False)
     return args, grid, mp, attr, bc
+
+
+def gaussian_random_field(nx, ny, dx, corr_len, std, seed):
+    """Seeded stationary Gaussian random field on an nx x ny grid of spacing dx
+    with squared-exponential covariance std^2 exp(-r^2 / corr_len^2).
+
+    Spectral synthesis: white noise filtered by the square root of the
+    covariance's spectral density exp(-k^2 corr_len^2 / 4), on a grid padded by
+    3 corr_len so the periodic FFT does not correlate opposite edges; the
+    sample is then shifted to zero mean and scaled to the requested standard
+    deviation.  (The reference's grf_gc! draws from Julia's global RNG and
+    min-max rescales; this one is reproducible from `seed`.)"""
+    pad = int(math.ceil(3.0 * corr_len / dx))
+    Nx, Ny = nx + 2 * pad, ny + 2 * pad
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((Nx, Ny))
+    kx = 2.0 * np.pi * np.fft.fftfreq(Nx, dx)
+    ky = 2.0 * np.pi * np.fft.rfftfreq(Ny, dx)
+    k2 = kx[:, None] ** 2 + ky[None, :] ** 2
+    f = np.fft.irfft2(np.fft.rfft2(w) * np.exp(-k2 * corr_len ** 2 / 8.0), s=(Nx, Ny))
+    f = f[pad:pad + nx, pad:pad + ny]
+    f = f - f.mean()
+    return f * (std / f.std())
+
+
+def _bilinear(field, x0, dx, y0, dy, x, y):
+    fx = np.asarray(x, dtype=np.float64)
+    fy = np.asarray(y, dtype=np.float64)
+    fx = np.clip((fx - x0) / dx, 0.0, field.shape[0] - 1.0)
+    fy = np.clip((fy - y0) / dy, 0.0, field.shape[1] - 1.0)
+    i = np.minimum(fx.astype(np.int64), field.shape[0] - 2)
+    j = np.minimum(fy.astype(np.int64), field.shape[1] - 2)
+    tx, ty = fx - i, fy - j
+    return ((1 - tx) * (1 - ty) * field[i, j] + tx * (1 - ty) * field[i + 1, j] +
+            (1 - tx) * ty * field[i, j + 1] + tx * ty * field[i + 1, j + 1])
+
+
+def slope_3d(*, h=0.5, length=400.0, width=200.0, angle_deg=35.0, depth=10.0,
+             footprint=(250.0, 125.0), footprint_x0=20.0, corr_len=20.0, grf_std=2.0,
+             phi_deg_range=(25.0, 35.0), cohesion_median=5.0e3, cohesion_sigma_ln=0.5,
+             rho=2000.0, E_mod=10.0e6, nu=0.3, seed=7, headroom=25.0, pad=2, eps="FP32",
+             cfl=0.2, k0_steps=2000, zeta_k0=0.1, runout_time=30.0, device="HIP"):
+    """Synthetic 3D slope failure in a slope-aligned frame (x downslope, y
+    across, z normal to the mean plane), Mohr-Coulomb.
+
+    The rigid bed is the plane z = 0 perturbed by a seeded Gaussian random field
+    (correlation corr_len, standard deviation grf_std); every grid node at or
+    below it is fixed.  A depth-thick sliding mass covers the footprint
+    (2x2x2 particles per cell, columns following the bed).  Each cell of the
+    initial mass gets its own material: friction ~ U(phi_deg_range), cohesion
+    ~ LogNormal(ln cohesion_median, cohesion_sigma_ln), drawn from `seed`.
+    Gravity acts along -z cos(angle) and +x sin(angle) (args.ext carries the
+    downslope component).  Walls: x = 0 and x = length fix vx, y = 0 and y =
+    width fix vy.  The returned args describe the K0 phase (k0_steps damped
+    elastic steps with a gravity ramp, Te = k0_steps dt); `runout_args` in the
+    layout is the undamped runout phase.  The defaults give 20,000,000
+    particles (250 x 125 m footprint, 10 m deep, h = 0.5 m).
+
+    Returns (args, grid, mp, attr, bc, layout)."""
+    rng = np.random.default_rng(seed + 1)
+    lp = h / 2
+    g = 9.8
+    ang = math.radians(angle_deg)
+    # bed on the node lattice
+    nx_n = int(round(length / h)) + 1 + 2 * pad
+    ny_n = int(round(width / h)) + 1 + 2 * pad
+    x1, y1 = -pad * h, -pad * h
+    bed = gaussian_random_field(nx_n, ny_n, h, corr_len, grf_std, seed)
+    zmin = math.floor((bed.min() - 2 * h) / h) * h - pad * h
+    zmax = bed.max() + depth + headroom
+    grid = UserGrid3D(ϵ=eps, x1=x1, x2=x1 + (nx_n - 1) * h, y1=y1, y2=y1 + (ny_n - 1) * h,
+                      z1=zmin, z2=zmin + math.ceil((zmax - zmin) / h) * h, dx=h, dy=h, dz=h,
+                      NIC=27)
+    # particles: lp lattice in x, y over the footprint; 2 * depth / h layers above the bed
+    fx0, fy0 = footprint_x0, (width - footprint[1]) / 2.0
+    px = fx0 + lp / 2 + lp * np.arange(int(round(footprint[0] / lp)))
+    py = fy0 + lp / 2 + lp * np.arange(int(round(footprint[1] / lp)))
+    nz = int(round(depth / lp))
+    PX, PY = np.meshgrid(px, py, indexing="ij")
+    zb = _bilinear(bed, x1, h, y1, h, PX.ravel(), PY.ravel())
+    ncol = zb.size
+    pts = np.empty((ncol * nz, 3), dtype=np.float64)
+    pts[:, 0] = np.repeat(PX.ravel(), nz)
+    pts[:, 1] = np.repeat(PY.ravel(), nz)
+    pts[:, 2] = np.repeat(zb, nz) + lp / 2 + lp * np.tile(np.arange(nz), ncol)
+    mp = UserParticle3D(ϵ=eps, NIC=27, dx=lp, dy=lp, dz=lp, ξ=pts,
+                        ρs=np.full(len(pts), rho))
+    # one material per initial cell
+    cell = np.floor((pts - np.array([x1, y1, zmin])) / h).astype(np.int64)
+    key = cell[:, 0] + nx_n * (cell[:, 1] + ny_n * cell[:, 2])
+    uniq, nid = np.unique(key, return_inverse=True)
+    nm = len(uniq)
+    phi = np.radians(rng.uniform(*phi_deg_range, size=nm))
+    coh = np.exp(rng.normal(math.log(cohesion_median), cohesion_sigma_ln, size=nm))
+    Ks = E_mod / (3 * (1 - 2 * nu))
+    Gs = E_mod / (2 * (1 + nu))
+    z = np.zeros(nm)
+    attr = UserProperty(ϵ=eps, nid=nid + 1, ν=np.full(nm, nu), Es=np.full(nm, E_mod),
+                        Gs=np.full(nm, Gs), Ks=np.full(nm, Ks), σt=z, ϕ=phi, ϕr=z, ψ=z,
+                        c=coh, cr=z, Hp=z)
+    # rigid bed and walls by node coordinate (reference 3D node order: y fastest)
+    gx, gy, gz = grid.ξ[:, 0], grid.ξ[:, 1], grid.ξ[:, 2]
+    bed_at_node = _bilinear(bed, x1, h, y1, h, gx, gy)
+    fixed = gz <= bed_at_node
+    vx_idx = (fixed | (gx <= 0.0) | (gx >= length)).astype(np.int64)
+    vy_idx = (fixed | (gy <= 0.0) | (gy >= width)).astype(np.int64)
+    vz_idx = fixed.astype(np.int64)
+    zv = np.zeros(grid.ni)
+    bc = UserVBoundary3D(ϵ=eps, vx_s_idx=vx_idx, vx_s_val=zv, vy_s_idx=vy_idx, vy_s_val=zv,
+                         vz_s_idx=vz_idx, vz_s_val=zv)
+    dt = cfl * h / p_wave_speed(Ks, Gs, rho)
+    gvec = dict(gravity_x=g * math.sin(ang))
+    common = dict(ΔT=dt, FLIP=1, PIC=0, constitutive="mohrcoulomb", basis="uGIMP", MVL=False,
+                  device=device, coupling="OS", scheme="MUSL", va="a",
+                  gravity=-g * math.cos(ang), project_path=".", ϵ=eps, progressbar=False,
+                  ext=gvec)
+    Te = k0_steps * dt
+    args = UserArgs3D(Ttol=Te, Te=Te, ζs=zeta_k0, project_name="slope3d_k0", **common)
+    runout_args = UserArgs3D(Ttol=Te + runout_time, Te=Te, ζs=0.0, project_name="slope3d",
+                             **common)
+    layout = dict(bed=bed, dt=dt, k0_steps=k0_steps, runout_args=runout_args, nmat=nm,
+                  footprint=(fx0, fy0, footprint[0], footprint[1]), seed=seed)
+    return args, grid, mp, attr, bc, layout

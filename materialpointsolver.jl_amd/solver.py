@@ -109,6 +109,11 @@ class HIPModel:
         a.mvl = 1 if args.MVL else 0
         a.flip, a.pic = float(args.FLIP), float(args.PIC)
         a.zeta_s, a.gravity, a.Te = float(args.ζs), float(args.gravity), float(args.Te)
+        # body force on the other axes: an extension carried in args.ext (the
+        # reference's user-extension slot); absent = the reference's gravity
+        ext = args.ext if isinstance(args.ext, dict) else {}
+        a.gravity_x = float(ext.get("gravity_x", 0.0))
+        a.gravity_y = float(ext.get("gravity_y", 0.0))
         return a
 
     def step(self, sargs, dt, Ti):

@@ -64,6 +64,9 @@ typedef struct oracle_model {
     /* ---- velocity boundary (src/types/boundary.jl) ---- */
     const int64_t *vx_idx, *vy_idx, *vz_idx; /* ni, != 0 -> fixed */
     const double *vx_val, *vy_val, *vz_val;
+    /* ---- extension: body force on the other axes (0 = the reference) ---- */
+    double gravity_x, gravity_y;
+    double Ti_now;         /* time of the current step (set by oracle_procedure) */
 } oracle_model;
 
 #define COL(a, n, i, k) ((a)[(int64_t)(k) * (n) + (i)])   /* (i,k) 0-based of (n, .) */
@@ -322,6 +325,10 @@ static inline void scatter_add(const oracle_model *m, double *a, double v) {
  * P2G_OS! (utils_OS.jl:424-452 2D, 461-496 3D)
  * ------------------------------------------------------------------------- */
 void oracle_p2g(oracle_model *m, double gravity) {
+    /* body force on the other axes (an extension of the reference, zero there) */
+    const double Te = m->Te, Ti = m->Ti_now;
+    const double gx = Ti < Te ? m->gravity_x / Te * Ti : m->gravity_x;
+    const double gy = Ti < Te ? m->gravity_y / Te * Ti : m->gravity_y;
     const int64_t np = m->np, ni = m->ni, NIC = m->nic, D = m->dim;
     #pragma omp parallel for if (par(m)) schedule(static)
     for (int64_t ix = 0; ix < np; ++ix) {
@@ -336,20 +343,23 @@ void oracle_p2g(oracle_model *m, double gravity) {
                 scatter_add(m, &COL(m->g_ps, ni, n, 0), Ni * COL(m->ps, np, ix, 0));
                 scatter_add(m, &COL(m->g_ps, ni, n, 1), Ni * COL(m->ps, np, ix, 1));
                 if (D == 2) {
-                    scatter_add(m, &COL(m->g_fs, ni, n, 0),
-                                -vol * (dNx * COL(m->sig, np, ix, 0) + dNy * COL(m->sig, np, ix, 3)));
+                    double fx = -vol * (dNx * COL(m->sig, np, ix, 0) + dNy * COL(m->sig, np, ix, 3));
+                    if (gx != 0.0) fx += NiM * gx;
+                    scatter_add(m, &COL(m->g_fs, ni, n, 0), fx);
                     scatter_add(m, &COL(m->g_fs, ni, n, 1),
                                 -vol * (dNy * COL(m->sig, np, ix, 1) + dNx * COL(m->sig, np, ix, 3)) +
                                     NiM * gravity);
                 } else {
                     double dNz = COL(m->dNz, np, ix, iy);
                     scatter_add(m, &COL(m->g_ps, ni, n, 2), Ni * COL(m->ps, np, ix, 2));
-                    scatter_add(m, &COL(m->g_fs, ni, n, 0),
-                                -vol * (dNx * COL(m->sig, np, ix, 0) + dNy * COL(m->sig, np, ix, 3) +
-                                        dNz * COL(m->sig, np, ix, 5)));
-                    scatter_add(m, &COL(m->g_fs, ni, n, 1),
-                                -vol * (dNy * COL(m->sig, np, ix, 1) + dNx * COL(m->sig, np, ix, 3) +
-                                        dNz * COL(m->sig, np, ix, 4)));
+                    double fx = -vol * (dNx * COL(m->sig, np, ix, 0) + dNy * COL(m->sig, np, ix, 3) +
+                                        dNz * COL(m->sig, np, ix, 5));
+                    double fy = -vol * (dNy * COL(m->sig, np, ix, 1) + dNx * COL(m->sig, np, ix, 3) +
+                                        dNz * COL(m->sig, np, ix, 4));
+                    if (gx != 0.0) fx += NiM * gx;
+                    if (gy != 0.0) fy += NiM * gy;
+                    scatter_add(m, &COL(m->g_fs, ni, n, 0), fx);
+                    scatter_add(m, &COL(m->g_fs, ni, n, 1), fy);
                     scatter_add(m, &COL(m->g_fs, ni, n, 2),
                                 -vol * (dNz * COL(m->sig, np, ix, 2) + dNx * COL(m->sig, np, ix, 5) +
                                         dNy * COL(m->sig, np, ix, 4)) +
@@ -955,6 +965,7 @@ void oracle_procedure(oracle_model *m, double dt, double Ti) {
     if (m->nthreads > 1) omp_set_num_threads(m->nthreads);
 #endif
     double G = Ti < m->Te ? m->gravity / m->Te * Ti : m->gravity;
+    m->Ti_now = Ti;
     oracle_resetgrid(m);
     oracle_resetmp(m);
     oracle_p2g(m, G);

@@ -77,6 +77,7 @@ class OracleModel(C.Structure):
         ("sigt", _D), ("Hp", _D), ("cr", _D),
         ("vx_idx", _I), ("vy_idx", _I), ("vz_idx", _I), ("vx_val", _D), ("vy_val", _D),
         ("vz_val", _D),
+        ("gravity_x", C.c_double), ("gravity_y", C.c_double), ("Ti_now", C.c_double),
     ]
 
 
@@ -216,6 +217,9 @@ class Oracle:
         m.nthreads = int(nthreads)
         m.flip, m.pic, m.zeta_s = float(args.FLIP), float(args.PIC), float(args.ζs)
         m.gravity, m.Te = float(args.gravity), float(args.Te)
+        ext = args.ext if isinstance(args.ext, dict) else {}
+        m.gravity_x = float(ext.get("gravity_x", 0.0))
+        m.gravity_y = float(ext.get("gravity_y", 0.0))
         m.nnx, m.nny = int(grid.nnx), int(grid.nny)
         m.nnz = int(grid.nnz) if D == 3 else 1
         m.ncx, m.ncy = int(grid.ncx), int(grid.ncy)

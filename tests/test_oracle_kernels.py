@@ -55,6 +55,26 @@ def test_p2g_conservation(mpm, orc, dim):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
+def test_p2g_body_force_extension(mpm, orc, dim):
+    # the body force on the other axes (extension; zero reproduces the reference)
+    # adds m_p g_a to axis a only, ramped with Ti / Te like gravity
+    o = orc.Oracle(*_block(mpm, dim))
+    _rand_state(o)
+    o.m.gravity_x, o.m.gravity_y, o.m.Te = 2.5, (-1.5 if dim == 3 else 0.0), 0.4
+    for Ti, fac in ((0.1, 0.25), (0.4, 1.0)):
+        o.m.Ti_now = Ti
+        o.resetgrid()
+        o.resetmp()
+        o.p2g(-9.8)
+        f = o.a["g_fs"].sum(axis=0)
+        m_tot = o.a["ms"].sum()
+        assert abs(f[0] - 2.5 * fac * m_tot) < 1e-9
+        if dim == 3:
+            assert abs(f[1] + 1.5 * fac * m_tot) < 1e-9
+        assert abs(f[-1] + 9.8 * m_tot) < 1e-9
+
+
+@pytest.mark.parametrize("dim", [2, 3])
 def test_p2g_internal_force_is_divergence(mpm, orc, dim):
     # a uniform stress field exerts no net force on interior nodes and pulls the
     # block's boundary nodes outward / inward with total moment sum_n x_n f_n = -V σ
