@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs (FETCH_SIZE, WRITE_SIZE)")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner report")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--ens-instances", type=int, default=256)
     ap.add_argument("--ens-steps", type=int, default=200)
@@ -206,6 +207,12 @@ def main():
               file=sys.stderr, flush=True)
 
     step_bytes = sum(ALGO_BYTES[k][0] * n + ALGO_BYTES[k][1] * n_active for k in ALGO_BYTES)
+    tuned = None
+    if not a.no_autotune:
+        best, table, Ti = mpm.autotune(model, sargs, dt, Ti, steps=40, warm=10)
+        tuned = {"best": best, "ms_per_step": {f"sort{k}_{'tiles' if t else 'plain'}": round(s * 1e3, 4)
+                                                for k, t, s in table}}
+        print(f"# autotune: {best}", file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(a, args, grid, mp, attr, bc)
@@ -263,6 +270,7 @@ def main():
             "algorithmic_bytes": dk["algo_bytes"],
         },
         "cpu_baseline": cpu,
+        "autotune": tuned,
     }
     if not a.no_ensemble:
         out["batched"] = ensemble_leg(a, mpm)

@@ -21,6 +21,7 @@ MPM_FP64, MPM_FP32 = 0, 1
 MPM_UGIMP, MPM_LINEAR = 0, 1
 MPM_MUSL = 0
 MPM_VA_A, MPM_VA_V = 0, 1
+MPM_OPT_SORT_INTERVAL, MPM_OPT_TILES = 0, 1
 CONSTITUTIVE = {"linearelastic": 0, "druckerprager": 1, "mohrcoulomb": 2}
 
 
@@ -86,6 +87,7 @@ _PROTOS = {
                                 C.c_uint64, _F]),
     "mpm_mlp_bench": (C.c_int, [_CTX, C.c_int64, C.c_int, _D]),
     "mpm_set_surrogate": (C.c_int, [_CTX, _CTX, C.c_double]),
+    "mpm_set_option": (C.c_int, [_CTX, C.c_int, C.c_int64]),
     "mpm_surrogate_stats": (C.c_int, [_CTX, _I64, _I64]),
     "mpm_return_log": (C.c_int, [_CTX, C.c_int64, C.c_int64]),
     "mpm_return_log_read": (C.c_int, [_CTX, _F, _F, C.c_int64, _I64]),

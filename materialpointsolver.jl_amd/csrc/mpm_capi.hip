@@ -1017,6 +1017,22 @@ const char* mpm_profile_kernel_name(int k) {
 
 int mpm_profile_kernel_count(void) { return kNumKernels; }
 
+int mpm_set_option(mpm_ctx* c, int option, int64_t value) {
+    if (!c) return MPM_E_ARG;
+    if (option == MPM_OPT_SORT_INTERVAL) {
+        if (c->cfg.sort_interval <= 0)
+            return c->fail(MPM_E_STATE, "mpm_set_option: the context was created without sorting");
+        if (value < 1 || value > (1 << 30)) return c->fail(MPM_E_ARG, "mpm_set_option: sort interval >= 1");
+        c->cfg.sort_interval = int32_t(value);
+    } else if (option == MPM_OPT_TILES) {
+        c->use_tiles = value != 0;
+    } else {
+        return c->fail(MPM_E_ARG, "mpm_set_option: unknown option");
+    }
+    c->drop_graphs();
+    return MPM_OK;
+}
+
 int mpm_set_surrogate(mpm_ctx* c, mpm_mlp* m, double tol) {
     if (!c) return MPM_E_ARG;
     return c->impl->set_surrogate(c, m, tol);

@@ -152,6 +152,11 @@ class HIPModel:
         self._check(self.lib.mpm_count_active_nodes(self.ctx, capi.iptr(out)))
         return int(out[0])
 
+    def set_option(self, option, value):
+        """Runtime option: "sort_interval" (>= 1) or "tiles" (bool)."""
+        code = {"sort_interval": capi.MPM_OPT_SORT_INTERVAL, "tiles": capi.MPM_OPT_TILES}[option]
+        self._check(self.lib.mpm_set_option(self.ctx, code, int(value)))
+
     # -- learned 3D Mohr-Coulomb return ----------------------------------------
     def return_log(self, capacity, stride=1):
         """Start logging exact 3D Mohr-Coulomb return-mapping samples."""

@@ -96,3 +96,46 @@ def test_packed_instances_evolve_independently(mpm):
               flush=True)
         assert np.abs(d_alone).max() > 1e-5  # it moved
         assert r <= 1e-8 and rs <= 1e-8
+
+
+def test_slope_k0_then_runout(mpm):
+    # the GRF slope at 4 m cells: damped K0 phase with the gravity ramp, then the
+    # undamped runout with graph-captured steps; mass moves downslope
+    args, grid, mp, attr, bc, lay = mpm.scenes.slope_3d(h=4.0, seed=3, k0_steps=200)
+    x0 = np.asarray(mp.ξ, dtype=np.float64)
+    dt = lay["dt"]
+    m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=10)
+    try:
+        Ti = m.run(m.step_args(args), dt, 0.0, lay["k0_steps"], graph=True)
+        k0 = m.download()
+        Ti = m.run(m.step_args(lay["runout_args"]), dt, Ti, 400, graph=True)
+        m.sync()
+        out = m.download()
+    finally:
+        m.close()
+    print(f"{mp.np} particles, {lay['nmat']} materials; K0 mean |dz| "
+          f"{np.abs(k0['xi'][:, 2] - x0[:, 2]).mean():.3e} m; runout mean dx "
+          f"{(out['xi'][:, 0] - x0[:, 0]).mean():.3e} m, max {(out['xi'][:, 0] - x0[:, 0]).max():.2f} m",
+          flush=True)
+    assert np.all(np.isfinite(out["xi"])) and np.all(np.isfinite(out["sigma"]))
+    assert (out["xi"][:, 0] - x0[:, 0]).mean() > 0.01  # slides downslope
+    # compressive mean stress at depth after K0
+    sm = out["sigma_m"]
+    assert np.median(sm) < 0
+
+
+def test_autotune_on_the_device(mpm):
+    args, grid, mp, attr, bc = mpm.scenes.column_collapse_3d(
+        h=0.02, width=0.16, height=0.08, box_xy=0.4, box_z=0.12, nsteps=100, eps="FP32")
+    m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=10)
+    try:
+        sa = m.step_args(args)
+        best, table, Ti = mpm.autotune(m, sa, float(args.ΔT), 0.0, sort_intervals=(5, 20),
+                                       steps=5, warm=2)
+        assert len(table) == 4 and best["sort_interval"] in (5, 20)
+        print("autotune:", best, [(k, t, f"{s * 1e6:.0f} us") for k, t, s in table], flush=True)
+        m.run(sa, float(args.ΔT), Ti, 5, graph=True)
+        st = m.download()
+        assert np.all(np.isfinite(st["xi"]))
+    finally:
+        m.close()

@@ -170,3 +170,13 @@ def test_fp32_flying_block(mpm, orc):
     # limit of FP32 velocities); positions and volumes stay at ~1e-7.
     tol = dict(xi=1e-6, vs=1e-5, sigma=5e-3, eps_s=5e-3, eps_q=1e-3, omega=1e-6)
     _compare(mpm, orc, _flying_block(mpm, 3, eps="FP32"), 60, tol, sort_interval=5)
+
+
+def test_fp64_body_force_extension(mpm, orc):
+    # gravity on the first axes (args.ext; not in the reference) in the parity mode
+    sc = _c3(mpm, Te=0.004)
+    sc[0].ext = {"gravity_x": 2.0, "gravity_y": -1.0}
+    _compare(mpm, orc, sc, 60, 1e-9, sort_interval=5)
+    sc2 = _c2(mpm)
+    sc2[0].ext = {"gravity_x": 3.0}
+    _compare(mpm, orc, sc2, 100, 1e-9)

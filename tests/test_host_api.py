@@ -104,3 +104,29 @@ def test_solver_call_sequence_and_clock(mpm, monkeypatch):
     assert [s[1][3] for s in steps[:3]] == [0.0, float(np.float32(2e-5)),
                                             float(np.float32(np.float32(2e-5) * 2))]
     assert names[-3:] == ["mpm_sync", "mpm_download_particles", "mpm_destroy"]
+
+
+def test_autotune_picks_the_fastest_and_leaves_it_set(mpm):
+    class FakeModel:
+        """Step cost depends on the options; records the calls."""
+        def __init__(self):
+            self.opts, self.now, self.calls = {}, 0.0, []
+
+        def set_option(self, k, v):
+            self.opts[k] = v
+            self.calls.append((k, v))
+
+        def run(self, sargs, dt, Ti, n, graph=False):
+            cost = {5: 3.0, 10: 1.0, 20: 1.5, 40: 2.0}[self.opts["sort_interval"]]
+            self.now += n * cost * (1.0 if self.opts["tiles"] else 4.0)
+            return Ti + n * dt
+
+        def sync(self):
+            pass
+
+    m = FakeModel()
+    best, table, Ti = mpm.autotune(m, None, 0.5, 0.0, steps=4, warm=1, clock=lambda: m.now)
+    assert best == dict(sort_interval=10, tiles=True)
+    assert m.opts == {"sort_interval": 10, "tiles": True}
+    assert len(table) == 8 and min(t[2] for t in table) == 1.0
+    assert Ti == 8 * 5 * 0.5  # every trial advanced the clock
