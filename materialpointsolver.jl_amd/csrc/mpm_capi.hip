@@ -289,6 +289,11 @@ struct Impl : ImplBase {
             if (e == hipSuccess) e = hipMemset(gp.f[d], 0, sizeof(R) * ni);
             if (e == hipSuccess) e = hipMemset(gp.bcv[d], 0, sizeof(R) * ni);
             if (e == hipSuccess) e = hipMemset(gp.du[d], 0, sizeof(R) * ni);
+            // gathers over LDS windows read nodes of zero weight too: keep every
+            // field finite from the start
+            if (e == hipSuccess) e = hipMemset(gp.vs[d], 0, sizeof(R) * ni);
+            if (e == hipSuccess) e = hipMemset(gp.vsT[d], 0, sizeof(R) * ni);
+            if (e == hipSuccess) e = hipMemset(gp.as[d], 0, sizeof(R) * ni);
         }
         if (e == hipSuccess) e = hipMemset(gp.bcf, 0, ni);
         if (e == hipSuccess) e = hipMemset(gp.cell_sm, 0, sizeof(R) * nc);

@@ -292,9 +292,10 @@ __device__ __forceinline__ void for_stencil(const GridGeom& g, const Stencil<DIM
 }
 
 // Same iteration as for_stencil, with the node given as an offset inside a
-// WW-wide local window whose corner is the stencil base.
+// local window (row stride WW, layer stride ZS) whose corner is the stencil base.
 template <int DIM, int S, typename R, typename F>
-__device__ __forceinline__ void for_stencil_local(const Stencil<DIM, S, R>& st, int WW, F&& body) {
+__device__ __forceinline__ void for_stencil_local(const Stencil<DIM, S, R>& st, int WW, int ZS,
+                                                  F&& body) {
     if constexpr (DIM == 2) {
         static_for<S>([&](auto a) {
             static_for<S>([&](auto b) {
@@ -307,7 +308,7 @@ __device__ __forceinline__ void for_stencil_local(const Stencil<DIM, S, R>& st, 
             static_for<S>([&](auto b) {
                 static_for<S>([&](auto a) {
                     R Nx = st.N[0][a], Ny = st.N[1][b], Nz = st.N[2][c];
-                    body((c * WW + b) * WW + a, Nx * Ny * Nz, st.dN[0][a] * Ny * Nz,
+                    body(c * ZS + b * WW + a, Nx * Ny * Nz, st.dN[0][a] * Ny * Nz,
                          st.dN[1][b] * Nx * Nz, st.dN[2][c] * Nx * Ny);
                 });
             });

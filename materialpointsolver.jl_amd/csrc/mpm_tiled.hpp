@@ -42,12 +42,19 @@ struct TileGeom {
     int ntiles;
 };
 
-// Window geometry of a tile: NB bases and WW nodes per axis.
+// Window geometry of a tile: NB bases and WW nodes per axis, x fastest with a
+// z stride ZS.  A full round touches, at one stencil slot, the nodes of a
+// 4x4x4 block of bases; with ZS = 64 that block's addresses fall on only 16 of
+// the 32 LDS banks (4 lanes per bank).  ZS = 68 (= 4 mod 32) puts the odd z
+// layers on the other 16 banks, so every bank serves exactly 2 lanes, the
+// minimum for 64 lanes.  The 4 padding words per z layer are never read.
 template <int DIM, int S, int TSH>
 struct Win {
     static constexpr int NB = (1 << TSH) + 2;
     static constexpr int WW = NB + S - 1;
-    static constexpr int WN = DIM == 3 ? WW * WW * WW : WW * WW;
+    static constexpr int ZS = (DIM == 3 && S == 3) ? WW * WW + ((4 - WW * WW) % 32 + 32) % 32
+                                                   : WW * WW;
+    static constexpr int WN = DIM == 3 ? ZS * WW : WW * WW;
     static constexpr int NBN = DIM == 3 ? NB * NB * NB : NB * NB;
 };
 
@@ -78,21 +85,33 @@ using lds_ptr = volatile __attribute__((address_space(3))) T*;
 
 // Window offsets of a stencil base (node offset wb, base offset bidx); false
 // if the base is outside the window.
-template <int DIM, int NB, int WW>
+template <int DIM, int NB, int WW, int ZS>
 __device__ __forceinline__ bool window_slot(const int* base, const int* w0, int& wb, int& bidx) {
     const int rx = base[0] - w0[0], ry = base[1] - w0[1], rz = DIM == 3 ? base[2] - w0[2] : 0;
     const bool inside = rx >= 0 && rx < NB && ry >= 0 && ry < NB && rz >= 0 && rz < NB;
-    wb = DIM == 3 ? (rz * WW + ry) * WW + rx : ry * WW + rx;
+    wb = DIM == 3 ? rz * ZS + ry * WW + rx : ry * WW + rx;
     bidx = DIM == 3 ? (rz * NB + ry) * NB + rx : ry * NB + rx;
     return inside;
 }
 
+// Window slot k -> node offsets (kx, ky, kz) inside the window; false for a
+// padding slot.
+template <int DIM, int WW, int ZS>
+__device__ __forceinline__ bool window_node(int k, int& kx, int& ky, int& kz) {
+    kz = DIM == 3 ? k / ZS : 0;
+    const int r = k - kz * ZS;
+    ky = r / WW;
+    kx = r - ky * WW;
+    return ky < WW;
+}
+
 // Stage NF grid fields of a tile's window into LDS (zero outside the grid).
-template <int DIM, int WW, int WN, int NF, typename R>
+template <int DIM, int WW, int ZS, int WN, int NF, typename R>
 __device__ __forceinline__ void stage_window(const GridGeom& g, const int* w0, R* const* fields,
                                              R* lds) {
     for (int k = threadIdx.x; k < WN; k += 64) {
-        const int kx = k % WW, ky = (k / WW) % WW, kz = DIM == 3 ? k / (WW * WW) : 0;
+        int kx, ky, kz;
+        if (!window_node<DIM, WW, ZS>(k, kx, ky, kz)) continue;
         const int gx = w0[0] + kx, gy = w0[1] + ky, gz = DIM == 3 ? w0[2] + kz : 0;
         const bool in = gx >= 0 && gx < g.nn[0] && gy >= 0 && gy < g.nn[1] &&
                         (DIM == 2 || (gz >= 0 && gz < g.nn[2]));
@@ -104,10 +123,12 @@ __device__ __forceinline__ void stage_window(const GridGeom& g, const int* w0, R
 
 // Flush a scatter window: global atomic adds of every non-zero node (only
 // window nodes inside the grid can be non-zero).
-template <int DIM, int WW, int WN, int NF, typename R>
+template <int DIM, int WW, int ZS, int WN, int NF, typename R>
 __device__ __forceinline__ void flush_window(const GridGeom& g, const int* w0, lds_ptr<R> win,
                                              R* const* fields) {
     for (int k = threadIdx.x; k < WN; k += 64) {
+        int kx, ky, kz;
+        if (!window_node<DIM, WW, ZS>(k, kx, ky, kz)) continue;
         R vals[NF];
         bool any = false;
 #pragma unroll
@@ -116,7 +137,6 @@ __device__ __forceinline__ void flush_window(const GridGeom& g, const int* w0, l
             any = any || vals[f] != R(0);
         }
         if (!any) continue;
-        const int kx = k % WW, ky = (k / WW) % WW, kz = DIM == 3 ? k / (WW * WW) : 0;
         const long long n = node_id(g, w0[0] + kx, w0[1] + ky, DIM == 3 ? w0[2] + kz : 0);
 #pragma unroll
         for (int f = 0; f < NF; ++f) atomic_add(&fields[f][n], vals[f]);
@@ -165,7 +185,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int* __restrict__ seg_begin, const int* __restrict__ seg_end,
     const int* __restrict__ count_ptr, int* flags) {
     using W = Win<DIM, S, TSH>;
-    constexpr int WW = W::WW, WN = W::WN, NB = W::NB;
+    constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
     constexpr int NF = 1 + 2 * DIM;
     __shared__ R win_[NF * WN];
     __shared__ unsigned claim_[W::NBN];
@@ -211,12 +231,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 sg[5] = pp.sig[5][i];
             }
             int wb, bidx;
-            const bool inside = window_slot<DIM, NB, WW>(st.base, w0, wb, bidx);
+            const bool inside = window_slot<DIM, NB, WW, ZS>(st.base, w0, wb, bidx);
             if (inside) claim[bidx] = tag;
             const bool won = inside && claim[bidx] == tag;
             tag += 64u;
             if (won) {
-                for_stencil_local<DIM, S, R>(st, WW, [&](int off, R N, R dNx, R dNy, R dNz) {
+                for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R dNx, R dNy, R dNz) {
                     if (N != R(0)) {
                         R c[NF];
                         p2g_values<DIM, R>(N, dNx, dNy, dNz, m, vol, pv, sg, G, c);
@@ -244,7 +264,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 });
             }
         }
-        flush_window<DIM, WW, WN, NF, R>(g, w0, win, fields);
+        flush_window<DIM, WW, ZS, WN, NF, R>(g, w0, win, fields);
     }
     raise_flag(flags, flag);
 }
@@ -261,7 +281,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int* __restrict__ seg_begin, const int* __restrict__ seg_end,
     const int* __restrict__ count_ptr) {
     using W = Win<DIM, S, TSH>;
-    constexpr int WW = W::WW, WN = W::WN, NB = W::NB;
+    constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
     constexpr int NG = 3 * DIM;  // gathered: vsT, as, vs
     __shared__ R gwin[NG * WN];
     __shared__ R win_[DIM * WN];
@@ -285,7 +305,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         int w0[3];
         tile_window_origin<DIM>(tg, tile, w0);
         __syncthreads();  // the previous tile's reads of gwin are done
-        stage_window<DIM, WW, WN, NG, R>(g, w0, gfields, gwin);
+        stage_window<DIM, WW, ZS, WN, NG, R>(g, w0, gfields, gwin);
         for (int k = threadIdx.x; k < DIM * WN; k += 64) win[k] = R(0);
         __syncthreads();
         for (int cur = beg; cur < end;) {
@@ -304,21 +324,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 continue;
             }
             int wb, bidx;
-            const bool inside = window_slot<DIM, NB, WW>(st.base, w0, wb, bidx);
+            const bool inside = window_slot<DIM, NB, WW, ZS>(st.base, w0, wb, bidx);
             R A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, C[3] = {0, 0, 0};
             if (inside) {
-                for_stencil_local<DIM, S, R>(st, WW, [&](int off, R N, R, R, R) {
-                    if (N != R(0)) {
-                        const R* L = gwin + wb + off;
-                        R vT[3], vA[3], vS[3];
+                // branch-free: zero weights add exact zeros (staged values are finite)
+                for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R, R, R) {
+                    const R* L = gwin + wb + off;
+                    R vT[3], vA[3], vS[3];
 #pragma unroll
-                        for (int d = 0; d < DIM; ++d) {
-                            vT[d] = L[d * WN];
-                            vA[d] = L[(DIM + d) * WN];
-                            vS[d] = L[(2 * DIM + d) * WN];
-                        }
-                        g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
+                    for (int d = 0; d < DIM; ++d) {
+                        vT[d] = L[d * WN];
+                        vA[d] = L[(DIM + d) * WN];
+                        vS[d] = L[(2 * DIM + d) * WN];
                     }
+                    g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
                 });
             } else {
                 for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
@@ -340,7 +359,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const bool won = inside && claim[bidx] == tag;
             tag += 64u;
             if (won) {
-                for_stencil_local<DIM, S, R>(st, WW, [&](int off, R N, R, R, R) {
+                for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R, R, R) {
                     if (N != R(0)) {
                         lds_ptr<R> L = win + wb + off;
                         R old[3];
@@ -361,7 +380,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 });
             }
         }
-        flush_window<DIM, WW, WN, DIM, R>(g, w0, win, pfields);
+        flush_window<DIM, WW, ZS, WN, DIM, R>(g, w0, win, pfields);
     }
 }
 
@@ -375,7 +394,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     StepArgs<R> sa, const int* __restrict__ tile_list, const int* __restrict__ seg_begin,
     const int* __restrict__ seg_end, const int* __restrict__ count_ptr, int* flags) {
     using W = Win<DIM, S, TSH>;
-    constexpr int WW = W::WW, WN = W::WN, NB = W::NB;
+    constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
     constexpr int NF = DIM * DIM;
     __shared__ R uwin[DIM * WN];
     R* ufields[3] = {gp.du[0], gp.du[1], DIM == 3 ? gp.du[2] : gp.du[1]};
@@ -388,7 +407,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         int w0[3];
         tile_window_origin<DIM>(tg, tile, w0);
         __syncthreads();
-        stage_window<DIM, WW, WN, DIM, R>(g, w0, ufields, uwin);
+        stage_window<DIM, WW, ZS, WN, DIM, R>(g, w0, ufields, uwin);
         __syncthreads();
         for (int i = beg + int(threadIdx.x); i < end; i += 64) {
             double x[3];
@@ -399,7 +418,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 continue;
             }
             int wb, bidx;
-            const bool inside = window_slot<DIM, NB, WW>(st.base, w0, wb, bidx);
+            const bool inside = window_slot<DIM, NB, WW, ZS>(st.base, w0, wb, bidx);
             R F[NF];
 #pragma unroll
             for (int k = 0; k < NF; ++k) F[k] = R(0);
@@ -416,8 +435,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 }
             };
             if (inside) {
-                for_stencil_local<DIM, S, R>(st, WW, [&](int off, R N, R dNx, R dNy, R dNz) {
-                    if (N != R(0)) {
+                // branch-free: a zero weight comes with zero gradients for uGIMP;
+                // the linear basis keeps the test (its gradient is not zero there)
+                for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R dNx, R dNy, R dNz) {
+                    if (S == 3 || N != R(0)) {
                         R u[3];
 #pragma unroll
                         for (int d = 0; d < DIM; ++d) u[d] = uwin[d * WN + wb + off];

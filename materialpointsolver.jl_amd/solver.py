@@ -13,6 +13,7 @@ argument.  Only ``"HIP"`` has an implementation: every other device raises.
 """
 from __future__ import annotations
 
+import os
 import time
 
 import numpy as np
@@ -280,27 +281,67 @@ def procedure(args, grid, mp, attr, bc, ΔT, Ti, coupling, scheme, va):
     mp.model.step(sargs, ΔT, Ti)
 
 
+class SnapshotWriter:
+    """The reference's HDF5 snapshot layout (src/solver.jl:53-115) in NumPy
+    containers (h5py is not available here): one ``group<k>.npz`` per snapshot
+    with the datasets stress, strain_s, eqstrain, ekstrain, eqrate, coords,
+    velocity_s, volume, mass_s, time, and ``meta.npz`` with FILE_NUM,
+    grid_coords, mp_coords0, nid and the vbc_* arrays, under
+    ``<project_path>/<project_name>/<project_name>_snapshots/``.  Arrays keep
+    the reference's (np, k) layout and dataset names."""
+
+    def __init__(self, args):
+        self.dir = os.path.join(args.project_path, args.project_name,
+                                f"{args.project_name}_snapshots")
+        if os.path.isdir(self.dir):
+            for f in os.listdir(self.dir):
+                if f.endswith(".npz"):
+                    os.remove(os.path.join(self.dir, f))
+        os.makedirs(self.dir, exist_ok=True)
+
+    def group(self, k, mp, Ti):
+        np.savez(os.path.join(self.dir, f"group{k}.npz"), stress=mp.σij, strain_s=mp.ϵijs,
+                 eqstrain=mp.ϵq, ekstrain=mp.ϵk, eqrate=mp.ϵv, coords=mp.ξ, velocity_s=mp.vs,
+                 volume=mp.Ω, mass_s=mp.ms, time=np.asarray(Ti))
+
+    def close(self, file_num, grid, mp, attr, bc):
+        extra = {}
+        if grid.dim == 3:
+            extra = dict(vbc_zs_idx=bc.vz_s_idx, vbc_zs_val=bc.vz_s_val)
+        np.savez(os.path.join(self.dir, "meta.npz"), FILE_NUM=np.asarray(file_num),
+                 grid_coords=grid.ξ, mp_coords0=mp.ξ0, nid=attr.nid, vbc_xs_idx=bc.vx_s_idx,
+                 vbc_xs_val=bc.vx_s_val, vbc_ys_idx=bc.vy_s_idx, vbc_ys_val=bc.vy_s_val, **extra)
+
+
 def submit_work(args, grid, mp, attr, bc, workflow):
     """submit_work! (src/solver.jl:37-131) for the HIP device: upload, the
-    ``while Ti < Ttol`` loop with ``Ti += ΔT`` in T2, final sync, download, free.
-    (The reference's host-side initmpstatus! only fills the per-slot basis
-    arrays, which the device recomputes every step.)"""
-    if args.hdf5:
-        raise NotImplementedError("HDF5 snapshots are not implemented in this package yet")
+    ``while Ti < Ttol`` loop with ``Ti += ΔT`` in T2, snapshots every
+    ``hdf5_step`` steps when ``args.hdf5`` (SnapshotWriter), final sync,
+    download, free.  (The reference's host-side initmpstatus! only fills the
+    per-slot basis arrays, which the device recomputes every step.)"""
     T2 = args.T2
     Ti = T2(0.0)
     ΔT = T2(args.ΔT)
     dev_grid, dev_mp, dev_attr, dev_bc = host2device(grid, mp, attr, bc, args.device,
                                                      sort_interval=args.sort_interval)
     try:
+        writer = SnapshotWriter(args) if args.hdf5 else None
+        group_id, switch = 1, 0
         args.start_time = time.time()
         while Ti < args.Ttol:
+            if writer is not None and (switch == args.hdf5_step or switch == 0):
+                device2host(args, mp, dev_mp, args.device)
+                writer.group(group_id, mp, Ti)
+                switch, group_id = 0, group_id + 1
             workflow(args, dev_grid, dev_mp, dev_attr, dev_bc, ΔT, Ti, args.coupling,
                      args.scheme, args.va)
             Ti = T2(Ti + ΔT)
+            switch += 1
             args.iter_num += 1
-        KAsync(dev_mp)
         args.end_time = time.time()
+        if writer is not None:
+            writer.close(group_id, grid, mp, attr, bc)
+        KAsync(dev_mp)
         device2host(args, mp, dev_mp, args.device, verbose=args.progressbar)
     finally:
         clean_device(dev_grid, dev_mp, dev_attr, dev_bc, args.device)

@@ -130,3 +130,31 @@ def test_autotune_picks_the_fastest_and_leaves_it_set(mpm):
     assert m.opts == {"sort_interval": 10, "tiles": True}
     assert len(table) == 8 and min(t[2] for t in table) == 1.0
     assert Ti == 8 * 5 * 0.5  # every trial advanced the clock
+
+
+def test_snapshot_layout_and_cadence(mpm, monkeypatch, tmp_path):
+    # groups at iterations 0, hdf5_step, 2 hdf5_step, ... with the reference's
+    # dataset names (src/solver.jl:53-115); the download precedes each group
+    rec = _Recorder()
+    monkeypatch.setattr(mpm._capi, "_lib", rec)
+    args, grid, mp, attr, bc = mpm.scenes.column_collapse_2d(
+        h=0.02, dt=2e-5, Ttol=1e-3, eps="FP32", project_name="snap", project_path=str(tmp_path))
+    args.hdf5, args.hdf5_step = True, 7
+    mpm.materialpointsolver(args, grid, mp, attr, bc)
+    n = args.iter_num
+    d = tmp_path / "snap" / "snap_snapshots"
+    groups = sorted(p.name for p in d.glob("group*.npz"))
+    want = len(range(0, n, 7))
+    assert len(groups) == want
+    g = np.load(d / "group1.npz")
+    assert set(g.files) == {"stress", "strain_s", "eqstrain", "ekstrain", "eqrate", "coords",
+                            "velocity_s", "volume", "mass_s", "time"}
+    assert g["coords"].shape == (mp.np, 2) and float(g["time"]) == 0.0
+    g2 = np.load(d / "group2.npz")
+    assert float(g2["time"]) == pytest.approx(7 * 2e-5, rel=1e-5)
+    meta = np.load(d / "meta.npz")
+    assert int(meta["FILE_NUM"]) == want + 1
+    assert meta["grid_coords"].shape == (grid.ni, 2) and meta["nid"].shape == (mp.np,)
+    names = [c[0] for c in rec.calls]
+    # one download per group plus the final one
+    assert names.count("mpm_download_particles") == want + 1
