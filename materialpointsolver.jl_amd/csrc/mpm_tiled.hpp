@@ -42,18 +42,15 @@ struct TileGeom {
     int ntiles;
 };
 
-// Window geometry of a tile: NB bases and WW nodes per axis, x fastest with a
-// z stride ZS.  A full round touches, at one stencil slot, the nodes of a
-// 4x4x4 block of bases; with ZS = 64 that block's addresses fall on only 16 of
-// the 32 LDS banks (4 lanes per bank).  ZS = 68 (= 4 mod 32) puts the odd z
-// layers on the other 16 banks, so every bank serves exactly 2 lanes, the
-// minimum for 64 lanes.  The 4 padding words per z layer are never read.
+// Window geometry of a tile: NB bases and WW nodes per axis, x fastest, z
+// stride ZS.  (A z stride padded to 4 mod 32, which spreads a 4x4x4 block of
+// bases over all LDS banks, measured slower on the 3D column: the larger
+// scatter window costs a resident wavefront per CU.  ZS stays WW * WW.)
 template <int DIM, int S, int TSH>
 struct Win {
     static constexpr int NB = (1 << TSH) + 2;
     static constexpr int WW = NB + S - 1;
-    static constexpr int ZS = (DIM == 3 && S == 3) ? WW * WW + ((4 - WW * WW) % 32 + 32) % 32
-                                                   : WW * WW;
+    static constexpr int ZS = WW * WW;
     static constexpr int WN = DIM == 3 ? ZS * WW : WW * WW;
     static constexpr int NBN = DIM == 3 ? NB * NB * NB : NB * NB;
 };
