@@ -33,7 +33,8 @@ METRIC = "particle-steps/s (3D DP column, %HBM peak); batched & surrogate vs pla
 ALGO_BYTES = {
     "k_p2g": (68, 28),
     "k_grid1": (0, 77),
-    "k_g2p1_p2g2": (64, 48),
+    "k_g2p1_p2g2": (64, 36),  # re-map gather (+ the scatter when not split)
+    "k_p2g2": (0, 12),        # re-map momentum scatter (its particle reads are overhead)
     "k_grid2": (0, 29),
     "k_g2p2": (208, 12),
 }
@@ -67,9 +68,10 @@ def parse():
 # (a = b + s c over three float4 arrays of TRIAD_BYTES each: reads 2x, writes 1x).
 TRIAD_BYTES = 256 << 20
 KERNEL_RE = {
-    "k_p2g": r"mpm::k_p2g(_tiled)?<",
+    "k_p2g": r"mpm::k_p2g<|mpm::k_scatter_tiled<\d, \d, \d, \w+, 0>",
     "k_grid1": r"mpm::k_grid1<",
-    "k_g2p1_p2g2": r"mpm::k_g2p1_p2g2(_tiled)?<",
+    "k_g2p1_p2g2": r"mpm::k_g2p1(_p2g2)?(_tiled)?<",
+    "k_p2g2": r"mpm::k_scatter_tiled<\d, \d, \d, \w+, 1>",
     "k_grid2": r"mpm::k_grid2<",
     "k_g2p2": r"mpm::k_g2p2(_tiled)?<",
 }

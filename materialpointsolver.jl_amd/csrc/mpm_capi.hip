@@ -25,10 +25,10 @@ std::string g_create_error = "no error";
 
 const char* const kKernelNames[] = {"k_p2g",  "k_grid1", "k_g2p1_p2g2", "k_grid2",
                                     "k_g2p2", "k_mvl2",  "k_sort",      "k_permute",
-                                    "k_learned_return"};
+                                    "k_learned_return", "k_p2g2"};
 constexpr int kNumKernels = int(sizeof(kKernelNames) / sizeof(kKernelNames[0]));
 enum { KID_P2G = 0, KID_GRID1, KID_G2P1, KID_GRID2, KID_G2P2, KID_MVL2, KID_SORT, KID_PERMUTE,
-       KID_LEARNED };
+       KID_LEARNED, KID_P2G2 };
 
 constexpr int kBlock = 256;
 
@@ -265,6 +265,8 @@ struct Impl : ImplBase {
         A(pp.nid, np, "nid");
         A(pp.pid, np, "pid");
         A(pp.p2c, np, "p2c");
+        if (cf.sort_interval > 0)
+            for (int a = 0; a < dim; ++a) A(pp.pm[a], np, "momentum scratch");
         A(gp.m, ni, "grid m");
         for (int d = 0; d < dim; ++d) {
             A(gp.p[d], ni, "grid p");
@@ -501,7 +503,7 @@ struct Impl : ImplBase {
         constexpr int TSH = DIM == 3 ? 2 : 3;
         if ((rc = c->launch(KID_P2G, [&] {
                  if (tiled)
-                     k_p2g_tiled<DIM, S, TSH, R><<<sort.tiled_grid(), 64, 0, s>>>(
+                     k_scatter_tiled<DIM, S, TSH, R, 0><<<sort.tiled_grid(), 64, 0, s>>>(
                          geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
                          sort.seg_begin, sort.seg_end, sort.tile_count, fl);
                  else
@@ -512,15 +514,25 @@ struct Impl : ImplBase {
                  k_grid1<DIM, R><<<gb, kBlock, 0, s>>>(geom, gp, sa, int(nc));
              })) != MPM_OK)
             return rc;
+        // MUSL re-map: tiled = gather kernel (shared window, 256 threads) then the
+        // momentum scatter; otherwise one fused kernel with global atomics
         if ((rc = c->launch(KID_G2P1, [&] {
                  if (tiled)
-                     k_g2p1_p2g2_tiled<DIM, S, TSH, R><<<sort.tiled_grid(), 64, 0, s>>>(
-                         geom, sort.tg, pp, gp, sa, sort.sorted_keys(), sort.tile_list,
-                         sort.seg_begin, sort.seg_end, sort.tile_count);
+                     k_g2p1_tiled<DIM, S, TSH, R><<<sort.tiled_grid(), 256, 0, s>>>(
+                         geom, sort.tg, pp, gp, sa, sort.tile_list, sort.seg_begin, sort.seg_end,
+                         sort.tile_count);
                  else
                      k_g2p1_p2g2<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, sa, n, fl);
              })) != MPM_OK)
             return rc;
+        if (tiled) {
+            if ((rc = c->launch(KID_P2G2, [&] {
+                     k_scatter_tiled<DIM, S, TSH, R, 1><<<sort.tiled_grid(), 64, 0, s>>>(
+                         geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
+                         sort.seg_begin, sort.seg_end, sort.tile_count, fl);
+                 })) != MPM_OK)
+                return rc;
+        }
         if ((rc = c->launch(KID_GRID2, [&] {
                  k_grid2<DIM, R><<<nblocks(ni), kBlock, 0, s>>>(geom, gp, R(dt));
              })) != MPM_OK)
@@ -528,7 +540,7 @@ struct Impl : ImplBase {
         auto g2p2 = [&](auto cm) {
             constexpr int CM = decltype(cm)::value;
             if (tiled)
-                k_g2p2_tiled<DIM, S, TSH, R, CM><<<sort.tiled_grid(), 64, 0, s>>>(
+                k_g2p2_tiled<DIM, S, TSH, R, CM><<<sort.tiled_grid(), 256, 0, s>>>(
                     geom, sort.tg, pp, gp, d_mats, sa, sort.tile_list, sort.seg_begin,
                     sort.seg_end, sort.tile_count, fl);
             else

@@ -82,6 +82,7 @@ struct PartPtrs {
     int* pid;         // original particle index (for downloads after sorting)
     int* p2c;         // step-start cell id (MVL only)
     R* de[6];         // strain increment of the step (deferred 3D Mohr-Coulomb return only)
+    R* pm[3];         // MUSL scratch: new momentum between the re-map's gather and scatter
 };
 
 template <typename R>

@@ -106,7 +106,7 @@ __device__ __forceinline__ bool window_node(int k, int& kx, int& ky, int& kz) {
 template <int DIM, int WW, int ZS, int WN, int NF, typename R>
 __device__ __forceinline__ void stage_window(const GridGeom& g, const int* w0, R* const* fields,
                                              R* lds) {
-    for (int k = threadIdx.x; k < WN; k += 64) {
+    for (int k = threadIdx.x; k < WN; k += blockDim.x) {
         int kx, ky, kz;
         if (!window_node<DIM, WW, ZS>(k, kx, ky, kz)) continue;
         const int gx = w0[0] + kx, gy = w0[1] + ky, gz = DIM == 3 ? w0[2] + kz : 0;
@@ -123,7 +123,7 @@ __device__ __forceinline__ void stage_window(const GridGeom& g, const int* w0, R
 template <int DIM, int WW, int ZS, int WN, int NF, typename R>
 __device__ __forceinline__ void flush_window(const GridGeom& g, const int* w0, lds_ptr<R> win,
                                              R* const* fields) {
-    for (int k = threadIdx.x; k < WN; k += 64) {
+    for (int k = threadIdx.x; k < WN; k += blockDim.x) {
         int kx, ky, kz;
         if (!window_node<DIM, WW, ZS>(k, kx, ky, kz)) continue;
         R vals[NF];
@@ -173,27 +173,35 @@ __device__ __forceinline__ void p2g_values(R N, R dNx, R dNy, R dNz, R m, R vol,
 }
 
 // ---------------------------------------------------------------------------
-// P2G #1 (P2G_OS!, utils_OS.jl:424-496)
+// Tile scatters.  MODE 0: P2G #1 (P2G_OS!, utils_OS.jl:424-496): mass,
+// momentum and internal + external force.  MODE 1: the momentum scatter of the
+// MUSL re-map (doublemapping2_OS!, utils_OS.jl:942-959) from the new particle
+// momentum that k_g2p1_tiled stored in pp.pm, with the step-start basis.
 // ---------------------------------------------------------------------------
-template <int DIM, int S, int TSH, typename R>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_p2g_tiled(
+template <int DIM, int S, int TSH, typename R, int MODE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_scatter_tiled(
     GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, Gravity<R> G,
     const unsigned* __restrict__ keys, const int* __restrict__ tile_list,
     const int* __restrict__ seg_begin, const int* __restrict__ seg_end,
     const int* __restrict__ count_ptr, int* flags) {
     using W = Win<DIM, S, TSH>;
     constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
-    constexpr int NF = 1 + 2 * DIM;
+    constexpr int NF = MODE == 0 ? 1 + 2 * DIM : DIM;
     __shared__ R win_[NF * WN];
     __shared__ unsigned claim_[W::NBN];
     lds_ptr<R> win = (lds_ptr<R>)win_;
     lds_ptr<unsigned> claim = (lds_ptr<unsigned>)claim_;
     R* fields[NF];
-    fields[0] = gp.m;
+    if constexpr (MODE == 0) {
+        fields[0] = gp.m;
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        fields[1 + d] = gp.p[d];
-        fields[1 + DIM + d] = gp.f[d];
+        for (int d = 0; d < DIM; ++d) {
+            fields[1 + d] = gp.p[d];
+            fields[1 + DIM + d] = gp.f[d];
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) fields[d] = gp.p[d];
     }
     unsigned tag = threadIdx.x;  // unique per lane and round
     int flag = 0;
@@ -214,19 +222,35 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             double x[3];
             load_x<DIM>(pp.x, i, x);
             Stencil<DIM, S, R> st;
-            if (!make_stencil<DIM, S, R>(g, x, st, flag)) continue;
-            const R m = pp.ms[i], vol = pp.vol[i];
+            int fl2 = 0;
+            if (!make_stencil<DIM, S, R>(g, x, st, MODE == 0 ? flag : fl2)) continue;
+            R m = R(0), vol = R(0);
             R pv[3] = {0, 0, 0}, sg[6] = {0, 0, 0, 0, 0, 0};
+            if constexpr (MODE == 0) {
+                m = pp.ms[i];
+                vol = pp.vol[i];
 #pragma unroll
-            for (int a = 0; a < DIM; ++a) pv[a] = m * pp.v[a][i];
-            sg[0] = pp.sig[0][i];
-            sg[1] = pp.sig[1][i];
-            sg[3] = pp.sig[3][i];
-            if constexpr (DIM == 3) {
-                sg[2] = pp.sig[2][i];
-                sg[4] = pp.sig[4][i];
-                sg[5] = pp.sig[5][i];
+                for (int a = 0; a < DIM; ++a) pv[a] = m * pp.v[a][i];
+                sg[0] = pp.sig[0][i];
+                sg[1] = pp.sig[1][i];
+                sg[3] = pp.sig[3][i];
+                if constexpr (DIM == 3) {
+                    sg[2] = pp.sig[2][i];
+                    sg[4] = pp.sig[4][i];
+                    sg[5] = pp.sig[5][i];
+                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < DIM; ++a) pv[a] = pp.pm[a][i];
             }
+            auto contrib = [&](R N, R dNx, R dNy, R dNz, R* c) {
+                if constexpr (MODE == 0) {
+                    p2g_values<DIM, R>(N, dNx, dNy, dNz, m, vol, pv, sg, G, c);
+                } else {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) c[d] = pv[d] * N;
+                }
+            };
             int wb, bidx;
             const bool inside = window_slot<DIM, NB, WW, ZS>(st.base, w0, wb, bidx);
             if (inside) claim[bidx] = tag;
@@ -236,7 +260,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R dNx, R dNy, R dNz) {
                     if (N != R(0)) {
                         R c[NF];
-                        p2g_values<DIM, R>(N, dNx, dNy, dNz, m, vol, pv, sg, G, c);
+                        contrib(N, dNx, dNy, dNz, c);
                         lds_ptr<R> L = win + wb + off;
                         R old[NF];
 #pragma unroll
@@ -253,7 +277,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R dNx, R dNy, R dNz) {
                     if (N != R(0)) {
                         R c[NF];
-                        p2g_values<DIM, R>(N, dNx, dNy, dNz, m, vol, pv, sg, G, c);
+                        contrib(N, dNx, dNy, dNz, c);
 #pragma unroll
                         for (int f = 0; f < NF; ++f) atomic_add(&fields[f][n], c[f]);
                     }
@@ -383,10 +407,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
 // ---------------------------------------------------------------------------
 // G2P #2 with the Δu window staged in LDS (G2P_OS!, utils_OS.jl:1025-1153),
-// then the strain / constitutive / MVL / position commit of g2p2_commit
+// then the strain / constitutive / MVL / position commit of g2p2_commit.
+// Four wavefronts share the window of a tile.
 // ---------------------------------------------------------------------------
 template <int DIM, int S, int TSH, typename R, int CM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p2_tiled(
+__global__ __launch_bounds__(256) void k_g2p2_tiled(
     GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, const Material<R>* __restrict__ mats,
     StepArgs<R> sa, const int* __restrict__ tile_list, const int* __restrict__ seg_begin,
     const int* __restrict__ seg_end, const int* __restrict__ count_ptr, int* flags) {
@@ -406,7 +431,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         __syncthreads();
         stage_window<DIM, WW, ZS, WN, DIM, R>(g, w0, ufields, uwin);
         __syncthreads();
-        for (int i = beg + int(threadIdx.x); i < end; i += 64) {
+        for (int i = beg + int(threadIdx.x); i < end; i += blockDim.x) {
             double x[3];
             load_x<DIM>(pp.x, i, x);
             Stencil<DIM, S, R> st;
@@ -456,6 +481,91 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
     }
     raise_flag(flags, flag);
+}
+
+// ---------------------------------------------------------------------------
+// MUSL re-map, gather half (doublemapping1(_a)_OS!, utils_OS.jl:698-908): four
+// wavefronts share one staged window of vsT, as, vs per tile and take the
+// tile's particles in order (no rounds: gathers do not conflict); the new
+// momentum m v goes to pp.pm for the momentum scatter k_scatter_tiled<..., 1>.
+// ---------------------------------------------------------------------------
+template <int DIM, int S, int TSH, typename R>
+__global__ __launch_bounds__(256) void k_g2p1_tiled(GridGeom g, TileGeom tg, PartPtrs<R> pp,
+                                                   GridPtrs<R> gp, StepArgs<R> sa,
+                                                   const int* __restrict__ tile_list,
+                                                   const int* __restrict__ seg_begin,
+                                                   const int* __restrict__ seg_end,
+                                                   const int* __restrict__ count_ptr) {
+    using W = Win<DIM, S, TSH>;
+    constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
+    constexpr int NG = 3 * DIM;  // gathered: vsT, as, vs
+    __shared__ R gwin[NG * WN];
+    R* gfields[NG];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        gfields[d] = gp.vsT[d];
+        gfields[DIM + d] = gp.as[d];
+        gfields[2 * DIM + d] = gp.vs[d];
+    }
+    int e0, e1, est;
+    xcd_chunk(*count_ptr, e0, e1, est);
+    for (int e = e0; e < e1; e += est) {
+        const int tile = tile_list[e];
+        const int beg = seg_begin[tile], end = seg_end[tile];
+        int w0[3];
+        tile_window_origin<DIM>(tg, tile, w0);
+        __syncthreads();  // the previous tile's reads of gwin are done
+        stage_window<DIM, WW, ZS, WN, NG, R>(g, w0, gfields, gwin);
+        __syncthreads();
+        for (int i = beg + int(threadIdx.x); i < end; i += blockDim.x) {
+            double x[3];
+            load_x<DIM>(pp.x, i, x);
+            Stencil<DIM, S, R> st;
+            int flag = 0;  // reported by the P2G #1 kernel
+            if (!make_stencil<DIM, S, R>(g, x, st, flag)) {
+#pragma unroll
+                for (int a = 0; a < DIM; ++a) {
+                    pp.dxp[a][i] = R(0);
+                    pp.pm[a][i] = R(0);
+                }
+                continue;
+            }
+            int wb, bidx;
+            const bool inside = window_slot<DIM, NB, WW, ZS>(st.base, w0, wb, bidx);
+            R A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, C[3] = {0, 0, 0};
+            if (inside) {
+                // branch-free: zero weights add exact zeros (staged values are finite)
+                for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R, R, R) {
+                    const R* L = gwin + wb + off;
+                    R vT[3], vA[3], vS[3];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        vT[d] = L[d * WN];
+                        vA[d] = L[(DIM + d) * WN];
+                        vS[d] = L[(2 * DIM + d) * WN];
+                    }
+                    g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
+                });
+            } else {
+                for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
+                    if (N != R(0)) {
+                        R vT[3], vA[3], vS[3];
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) {
+                            vT[d] = gp.vsT[d][n];
+                            vA[d] = gp.as[d][n];
+                            vS[d] = gp.vs[d][n];
+                        }
+                        g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
+                    }
+                });
+            }
+            R pv[3] = {0, 0, 0};
+            g2p1_commit<DIM, R>(pp, sa, i, A, B, C, pv);
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) pp.pm[a][i] = pv[a];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
