@@ -140,17 +140,6 @@ __device__ __forceinline__ void flush_window(const GridGeom& g, const int* w0, l
     }
 }
 
-// One scatter round from position `cur` of the sorted order: the particles
-// sharing the tile and rank of particle `cur` (a prefix of the next 64, since
-// the keys are sorted).  Returns their number; `active` tells whether this
-// lane's particle cur + lane is one of them.
-__device__ __forceinline__ int rank_round(const unsigned* __restrict__ keys, int cur, int end,
-                                          bool& active) {
-    const unsigned grp = keys[cur] >> kLocalBits;
-    const int q = cur + int(threadIdx.x);
-    active = q < end && (keys[q] >> kLocalBits) == grp;
-    return __popcll(__ballot(active));
-}
 
 // P2G #1 contribution of one particle to one node: m, p[DIM], f[DIM]
 // (utils_OS.jl:466-494).
@@ -213,196 +202,121 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         int w0[3];
         tile_window_origin<DIM>(tg, tile, w0);
         for (int k = threadIdx.x; k < NF * WN; k += 64) win[k] = R(0);
-        for (int cur = beg; cur < end;) {
-            bool active;
-            const int cnt = rank_round(keys, cur, end, active);
-            const int i = cur + int(threadIdx.x);
-            cur += cnt;
-            if (!active) continue;
+        // Rounds are software-pipelined: the keys of the next round come from a
+        // two-register sliding window over the sorted keys (a round has at most
+        // 64 particles, so the next round lies in [cur, cur + 128)), and the next
+        // round's particle data is loaded before this round's LDS scatter.
+        const int lane = threadIdx.x;
+        auto member = [&](unsigned ka, int c0, bool& act) -> int {
+            const unsigned grp = unsigned(__shfl(int(ka), 0)) >> kLocalBits;
+            act = (c0 + lane < end) && (ka >> kLocalBits) == grp;
+            return __popcll(__ballot(act));
+        };
+        struct Pre {
             double x[3];
-            load_x<DIM>(pp.x, i, x);
-            Stencil<DIM, S, R> st;
-            int fl2 = 0;
-            if (!make_stencil<DIM, S, R>(g, x, st, MODE == 0 ? flag : fl2)) continue;
-            R m = R(0), vol = R(0);
-            R pv[3] = {0, 0, 0}, sg[6] = {0, 0, 0, 0, 0, 0};
+            R m, vol, v[3], sg[6];
+        };
+        auto load_pre = [&](int i, bool act, Pre& P) {
+            if (!act) return;
+            load_x<DIM>(pp.x, i, P.x);
             if constexpr (MODE == 0) {
-                m = pp.ms[i];
-                vol = pp.vol[i];
+                P.m = pp.ms[i];
+                P.vol = pp.vol[i];
 #pragma unroll
-                for (int a = 0; a < DIM; ++a) pv[a] = m * pp.v[a][i];
-                sg[0] = pp.sig[0][i];
-                sg[1] = pp.sig[1][i];
-                sg[3] = pp.sig[3][i];
+                for (int a = 0; a < DIM; ++a) P.v[a] = pp.v[a][i];
+                P.sg[0] = pp.sig[0][i];
+                P.sg[1] = pp.sig[1][i];
+                P.sg[3] = pp.sig[3][i];
                 if constexpr (DIM == 3) {
-                    sg[2] = pp.sig[2][i];
-                    sg[4] = pp.sig[4][i];
-                    sg[5] = pp.sig[5][i];
+                    P.sg[2] = pp.sig[2][i];
+                    P.sg[4] = pp.sig[4][i];
+                    P.sg[5] = pp.sig[5][i];
                 }
             } else {
 #pragma unroll
-                for (int a = 0; a < DIM; ++a) pv[a] = pp.pm[a][i];
+                for (int a = 0; a < DIM; ++a) P.v[a] = pp.pm[a][i];
             }
-            auto contrib = [&](R N, R dNx, R dNy, R dNz, R* c) {
-                if constexpr (MODE == 0) {
-                    p2g_values<DIM, R>(N, dNx, dNy, dNz, m, vol, pv, sg, G, c);
-                } else {
+        };
+        int cur = beg;
+        unsigned kA = cur + lane < end ? keys[cur + lane] : 0u;
+        unsigned kB = cur + 64 + lane < end ? keys[cur + 64 + lane] : 0u;
+        bool act = false;
+        int cnt = cur < end ? member(kA, cur, act) : 0;
+        Pre P{};
+        load_pre(cur + lane, act, P);
+        while (cur < end) {
+            const int src = (cnt + lane) & 63;
+            const unsigned a2 = unsigned(__shfl(int(kA), src)), b2 = unsigned(__shfl(int(kB), src));
+            const unsigned kAn = (cnt + lane < 64) ? a2 : b2;
+            const int curn = cur + cnt;
+            const unsigned kBn = curn + 64 + lane < end ? keys[curn + 64 + lane] : 0u;
+            bool actn = false;
+            const int cntn = curn < end ? member(kAn, curn, actn) : 0;
+            Pre Pn{};
+            load_pre(curn + lane, actn, Pn);
+            if (act) {
+                const int i = cur + lane;
+                Stencil<DIM, S, R> st;
+                int fl2 = 0;
+                if (make_stencil<DIM, S, R>(g, P.x, st, MODE == 0 ? flag : fl2)) {
+                    R pv[3] = {0, 0, 0};
+                    const R m = P.m, vol = P.vol;
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) c[d] = pv[d] * N;
+                    for (int a = 0; a < DIM; ++a) pv[a] = MODE == 0 ? m * P.v[a] : P.v[a];
+                    auto contrib = [&](R N, R dNx, R dNy, R dNz, R* c) {
+                        if constexpr (MODE == 0) {
+                            p2g_values<DIM, R>(N, dNx, dNy, dNz, m, vol, pv, P.sg, G, c);
+                        } else {
+#pragma unroll
+                            for (int d = 0; d < DIM; ++d) c[d] = pv[d] * N;
+                        }
+                    };
+                    int wb, bidx;
+                    const bool inside = window_slot<DIM, NB, WW, ZS>(st.base, w0, wb, bidx);
+                    if (inside) claim[bidx] = tag;
+                    const bool won = inside && claim[bidx] == tag;
+                    tag += 64u;
+                    if (won) {
+                        for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R dNx, R dNy, R dNz) {
+                            if (N != R(0)) {
+                                R c[NF];
+                                contrib(N, dNx, dNy, dNz, c);
+                                lds_ptr<R> L = win + wb + off;
+                                R old[NF];
+#pragma unroll
+                                for (int f = 0; f < NF; ++f) old[f] = L[f * WN];
+#pragma unroll
+                                for (int f = 0; f < NF; ++f) L[f * WN] = old[f] + c[f];
+                            }
+                            // keep each slot's arithmetic next to its LDS update
+                            __builtin_amdgcn_sched_barrier(0);
+                        });
+                    } else {
+                        // drifted out of the window, or shares its base with another
+                        // particle of the round
+                        for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R dNx, R dNy, R dNz) {
+                            if (N != R(0)) {
+                                R c[NF];
+                                contrib(N, dNx, dNy, dNz, c);
+#pragma unroll
+                                for (int f = 0; f < NF; ++f) atomic_add(&fields[f][n], c[f]);
+                            }
+                            __builtin_amdgcn_sched_barrier(0);
+                        });
+                    }
+                    (void)i;
                 }
-            };
-            int wb, bidx;
-            const bool inside = window_slot<DIM, NB, WW, ZS>(st.base, w0, wb, bidx);
-            if (inside) claim[bidx] = tag;
-            const bool won = inside && claim[bidx] == tag;
-            tag += 64u;
-            if (won) {
-                for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R dNx, R dNy, R dNz) {
-                    if (N != R(0)) {
-                        R c[NF];
-                        contrib(N, dNx, dNy, dNz, c);
-                        lds_ptr<R> L = win + wb + off;
-                        R old[NF];
-#pragma unroll
-                        for (int f = 0; f < NF; ++f) old[f] = L[f * WN];
-#pragma unroll
-                        for (int f = 0; f < NF; ++f) L[f * WN] = old[f] + c[f];
-                    }
-                    // keep each slot's arithmetic next to its LDS update
-                    __builtin_amdgcn_sched_barrier(0);
-                });
-            } else {
-                // drifted out of the window, or shares its base with another
-                // particle of the round
-                for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R dNx, R dNy, R dNz) {
-                    if (N != R(0)) {
-                        R c[NF];
-                        contrib(N, dNx, dNy, dNz, c);
-#pragma unroll
-                        for (int f = 0; f < NF; ++f) atomic_add(&fields[f][n], c[f]);
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
-                });
             }
+            P = Pn;
+            act = actn;
+            cnt = cntn;
+            cur = curn;
+            kA = kAn;
+            kB = kBn;
         }
         flush_window<DIM, WW, ZS, WN, NF, R>(g, w0, win, fields);
     }
     raise_flag(flags, flag);
-}
-
-// ---------------------------------------------------------------------------
-// MUSL re-map: gather vsT, as, vs from the staged window, update the particle
-// velocity, scatter its new momentum into a second window
-// (doublemapping1(_a)_OS! + doublemapping2_OS!, utils_OS.jl:698-959)
-// ---------------------------------------------------------------------------
-template <int DIM, int S, int TSH, typename R>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_g2p1_p2g2_tiled(
-    GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, StepArgs<R> sa,
-    const unsigned* __restrict__ keys, const int* __restrict__ tile_list,
-    const int* __restrict__ seg_begin, const int* __restrict__ seg_end,
-    const int* __restrict__ count_ptr) {
-    using W = Win<DIM, S, TSH>;
-    constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
-    constexpr int NG = 3 * DIM;  // gathered: vsT, as, vs
-    __shared__ R gwin[NG * WN];
-    __shared__ R win_[DIM * WN];
-    __shared__ unsigned claim_[W::NBN];
-    lds_ptr<R> win = (lds_ptr<R>)win_;
-    lds_ptr<unsigned> claim = (lds_ptr<unsigned>)claim_;
-    R* gfields[NG];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        gfields[d] = gp.vsT[d];
-        gfields[DIM + d] = gp.as[d];
-        gfields[2 * DIM + d] = gp.vs[d];
-    }
-    R* pfields[3] = {gp.p[0], gp.p[1], DIM == 3 ? gp.p[2] : gp.p[1]};
-    unsigned tag = threadIdx.x;
-    int e0, e1, est;
-    xcd_chunk(*count_ptr, e0, e1, est);
-    for (int e = e0; e < e1; e += est) {
-        const int tile = tile_list[e];
-        const int beg = seg_begin[tile], end = seg_end[tile];
-        int w0[3];
-        tile_window_origin<DIM>(tg, tile, w0);
-        __syncthreads();  // the previous tile's reads of gwin are done
-        stage_window<DIM, WW, ZS, WN, NG, R>(g, w0, gfields, gwin);
-        for (int k = threadIdx.x; k < DIM * WN; k += 64) win[k] = R(0);
-        __syncthreads();
-        for (int cur = beg; cur < end;) {
-            bool active;
-            const int cnt = rank_round(keys, cur, end, active);
-            const int i = cur + int(threadIdx.x);
-            cur += cnt;
-            if (!active) continue;
-            double x[3];
-            load_x<DIM>(pp.x, i, x);
-            Stencil<DIM, S, R> st;
-            int flag = 0;  // reported by k_p2g_tiled
-            if (!make_stencil<DIM, S, R>(g, x, st, flag)) {
-#pragma unroll
-                for (int a = 0; a < DIM; ++a) pp.dxp[a][i] = R(0);
-                continue;
-            }
-            int wb, bidx;
-            const bool inside = window_slot<DIM, NB, WW, ZS>(st.base, w0, wb, bidx);
-            R A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, C[3] = {0, 0, 0};
-            if (inside) {
-                // branch-free: zero weights add exact zeros (staged values are finite)
-                for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R, R, R) {
-                    const R* L = gwin + wb + off;
-                    R vT[3], vA[3], vS[3];
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d) {
-                        vT[d] = L[d * WN];
-                        vA[d] = L[(DIM + d) * WN];
-                        vS[d] = L[(2 * DIM + d) * WN];
-                    }
-                    g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
-                });
-            } else {
-                for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
-                    if (N != R(0)) {
-                        R vT[3], vA[3], vS[3];
-#pragma unroll
-                        for (int d = 0; d < DIM; ++d) {
-                            vT[d] = gp.vsT[d][n];
-                            vA[d] = gp.as[d][n];
-                            vS[d] = gp.vs[d][n];
-                        }
-                        g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
-                    }
-                });
-            }
-            R pv[3] = {0, 0, 0};
-            g2p1_commit<DIM, R>(pp, sa, i, A, B, C, pv);
-            if (inside) claim[bidx] = tag;
-            const bool won = inside && claim[bidx] == tag;
-            tag += 64u;
-            if (won) {
-                for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R, R, R) {
-                    if (N != R(0)) {
-                        lds_ptr<R> L = win + wb + off;
-                        R old[3];
-#pragma unroll
-                        for (int d = 0; d < DIM; ++d) old[d] = L[d * WN];
-#pragma unroll
-                        for (int d = 0; d < DIM; ++d) L[d * WN] = old[d] + pv[d] * N;
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
-                });
-            } else {
-                for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
-                    if (N != R(0)) {
-#pragma unroll
-                        for (int d = 0; d < DIM; ++d) atomic_add(&gp.p[d][n], pv[d] * N);
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
-                });
-            }
-        }
-        flush_window<DIM, WW, ZS, WN, DIM, R>(g, w0, win, pfields);
-    }
 }
 
 // ---------------------------------------------------------------------------
