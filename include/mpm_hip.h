@@ -46,7 +46,7 @@ enum {
 
 enum { MPM_FP64 = 0, MPM_FP32 = 1 };                 /* precision */
 enum { MPM_UGIMP = 0, MPM_LINEAR = 1 };               /* basis */
-enum { MPM_MUSL = 0 };                                /* scheme */
+enum { MPM_MUSL = 0, MPM_USL = 1, MPM_USF = 2 };     /* scheme */
 enum { MPM_VA_A = 0, MPM_VA_V = 1 };                  /* va */
 enum { MPM_LINEARELASTIC = 0, MPM_DRUCKERPRAGER = 1, MPM_MOHRCOULOMB = 2 }; /* constitutive */
 
@@ -70,9 +70,16 @@ typedef struct mpm_config {
 } mpm_config;
 
 /* Per-call switches: the fields of Args2D/Args3D that procedure! reads
- * (src/types/modelargs.jl:31-58,122-149; src/solvers/OS_MUSL.jl:12-102). */
+ * (src/types/modelargs.jl:31-58,122-149; src/solvers/OS_MUSL.jl:12-102,
+ * OS_USL.jl:102-144,325-366, OS_USF.jl:11-97). */
 typedef struct mpm_args {
-    int32_t scheme;        /* MPM_MUSL only (args.scheme) */
+    int32_t scheme;        /* args.scheme: MPM_MUSL, MPM_USL or MPM_USF.  A USF step updates
+                              the stress first, with the previous step's nodal displacement;
+                              its position update is committed by the next step's stress
+                              update (downloads add it), so the state after n USF steps is
+                              the reference's.  The first USF step of a context (or after
+                              steps of another scheme) applies the nodal displacement held
+                              on the grid: zero on a fresh context, as in the reference. */
     int32_t va;            /* MPM_VA_A | MPM_VA_V (args.va) */
     int32_t constitutive;  /* MPM_LINEARELASTIC | MPM_DRUCKERPRAGER | MPM_MOHRCOULOMB
                               (3D Mohr-Coulomb is defined by this library: a principal-space

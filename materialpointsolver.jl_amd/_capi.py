@@ -19,7 +19,8 @@ ERRORS = {-1: "MPM_E_ARG", -2: "MPM_E_UNSUPPORTED", -3: "MPM_E_HIP", -4: "MPM_E_
           -5: "MPM_E_STATE", -6: "MPM_E_RANGE"}
 MPM_FP64, MPM_FP32 = 0, 1
 MPM_UGIMP, MPM_LINEAR = 0, 1
-MPM_MUSL = 0
+MPM_MUSL, MPM_USL, MPM_USF = 0, 1, 2
+SCHEME = {"MUSL": MPM_MUSL, "USL": MPM_USL, "USF": MPM_USF}
 MPM_VA_A, MPM_VA_V = 0, 1
 MPM_OPT_SORT_INTERVAL, MPM_OPT_TILES = 0, 1
 CONSTITUTIVE = {"linearelastic": 0, "druckerprager": 1, "mohrcoulomb": 2}

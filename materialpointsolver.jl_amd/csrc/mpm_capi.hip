@@ -1,6 +1,6 @@
 // mpm_capi.hip -- C ABI of libmpm_hip.so (declared in include/mpm_hip.h).
 //
-// Host side of the gfx950 MUSL step: context and device buffers, uploads from
+// Host side of the gfx950 MUSL / USL / USF step: context and device buffers, uploads from
 // the reference's Julia layouts, the per-step launch sequence, downloads in
 // the caller's particle order, HIP-event kernel timing and the triad probe.
 #include <hip/hip_runtime.h>
@@ -25,10 +25,13 @@ std::string g_create_error = "no error";
 
 const char* const kKernelNames[] = {"k_p2g",  "k_grid1", "k_g2p1_p2g2", "k_grid2",
                                     "k_g2p2", "k_mvl2",  "k_sort",      "k_permute",
-                                    "k_learned_return", "k_p2g2"};
+                                    "k_learned_return", "k_p2g2", "k_mvl1"};
 constexpr int kNumKernels = int(sizeof(kKernelNames) / sizeof(kKernelNames[0]));
 enum { KID_P2G = 0, KID_GRID1, KID_G2P1, KID_GRID2, KID_G2P2, KID_MVL2, KID_SORT, KID_PERMUTE,
-       KID_LEARNED, KID_P2G2 };
+       KID_LEARNED, KID_P2G2, KID_MVL1 };
+// parts of a step: MUSL/USL run FULL; USF runs HEAD (the stress update of the
+// previous step's displacement), then the sort when due, then BODY
+enum { PH_FULL = 0, PH_HEAD = 1, PH_BODY = 2 };
 
 constexpr int kBlock = 256;
 
@@ -158,6 +161,16 @@ __global__ __launch_bounds__(256) void k_unpermute(const T* __restrict__ src,
     if (i < np) dst[pid[i]] = double(src[i]);
 }
 
+// dst[pid[i]] = x[i] + dx[i]: positions with a pending USF increment
+template <typename R>
+__global__ __launch_bounds__(256) void k_unpermute_add(const double* __restrict__ x,
+                                                       const R* __restrict__ dx,
+                                                       const int* __restrict__ pid,
+                                                       double* __restrict__ dst, int np) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) dst[pid[i]] = x[i] + double(dx[i]);
+}
+
 // σm = (σxx + σyy + σzz) * 0.333333 of the stored stress (liE!/dpP!/mcP! all
 // end with this assignment, linearelastic.jl:61-62), scattered to caller order.
 template <typename R>
@@ -195,6 +208,9 @@ struct Impl : ImplBase {
     float sur_tol = 1e-3f;
     unsigned long long* d_sur_stats = nullptr;  // [plastic, fallbacks]
     bool have_de = false;
+    // USF: the last step's position increment (dxp) and nodal displacement are
+    // waiting for the next step's stress update
+    bool usf_pending = false;
     // return-mapping sample log
     float *log_x = nullptr, *log_y = nullptr;
     unsigned long long* d_log_count = nullptr;
@@ -382,6 +398,7 @@ struct Impl : ImplBase {
         if (e != hipSuccess) return c->hip_fail(e, "upload nid/pid");
         sort.tiles_valid = false;  // caller order: sort again before the next step
         c->resort = true;
+        usf_pending = false;
         return MPM_OK;
     }
 
@@ -462,7 +479,7 @@ struct Impl : ImplBase {
     }
 
     template <int DIM, int S>
-    int step_impl(mpm_ctx* c, const mpm_args& ar, double dt, double Ti) {
+    int step_impl(mpm_ctx* c, const mpm_args& ar, double dt, double Ti, int phase) {
         const mpm_config& cf = c->cfg;
         // OS_MUSL.jl:70 gravity ramp and :88 plasticity gate, evaluated in FP64
         double G = Ti < ar.Te ? ar.gravity / ar.Te * Ti : ar.gravity;
@@ -479,6 +496,10 @@ struct Impl : ImplBase {
         sa.pic = R(ar.pic);
         sa.zeta = R(ar.zeta_s);
         sa.va = ar.va;
+        sa.gmode = ar.va == MPM_VA_V ? 1 : (ar.scheme == MPM_MUSL ? 0 : 2);
+        sa.remap = ar.scheme == MPM_MUSL ? 1 : 0;
+        // a USF context's first stress update has no position increment to commit
+        sa.commit_dx = (phase != PH_HEAD || usf_pending) ? 1 : 0;
         sa.plastic = Ti >= ar.Te ? 1 : 0;
         sa.constitutive = ar.constitutive;
         sa.mvl = ar.mvl;
@@ -501,50 +522,71 @@ struct Impl : ImplBase {
         (void)cf;
         const bool tiled = sort.enabled && sort.tiles_valid && c->use_tiles;
         constexpr int TSH = DIM == 3 ? 2 : 3;
-        if ((rc = c->launch(KID_P2G, [&] {
-                 if (tiled)
-                     k_scatter_tiled<DIM, S, TSH, R, 0><<<sort.tiled_grid(), 64, 0, s>>>(
-                         geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
-                         sort.seg_begin, sort.seg_end, sort.tile_count, fl);
-                 else
-                     k_p2g<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, Gv, n, fl);
-             })) != MPM_OK)
-            return rc;
-        if ((rc = c->launch(KID_GRID1, [&] {
-                 k_grid1<DIM, R><<<gb, kBlock, 0, s>>>(geom, gp, sa, int(nc));
-             })) != MPM_OK)
-            return rc;
-        // MUSL re-map: tiled = gather kernel (shared window, 256 threads) then the
-        // momentum scatter; otherwise one fused kernel with global atomics
-        if ((rc = c->launch(KID_G2P1, [&] {
-                 if (tiled)
-                     k_g2p1_tiled<DIM, S, TSH, R><<<sort.tiled_grid(), 256, 0, s>>>(
-                         geom, sort.tg, pp, gp, sa, sort.tile_list, sort.seg_begin, sort.seg_end,
-                         sort.tile_count);
-                 else
-                     k_g2p1_p2g2<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, sa, n, fl);
-             })) != MPM_OK)
-            return rc;
-        if (tiled) {
-            if ((rc = c->launch(KID_P2G2, [&] {
-                     k_scatter_tiled<DIM, S, TSH, R, 1><<<sort.tiled_grid(), 64, 0, s>>>(
-                         geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
-                         sort.seg_begin, sort.seg_end, sort.tile_count, fl);
+        if (phase != PH_HEAD) {
+            if ((rc = c->launch(KID_P2G, [&] {
+                     if (tiled)
+                         k_scatter_tiled<DIM, S, TSH, R, 0><<<sort.tiled_grid(), 64, 0, s>>>(
+                             geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
+                             sort.seg_begin, sort.seg_end, sort.tile_count, fl);
+                     else
+                         k_p2g<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, Gv, n, fl);
                  })) != MPM_OK)
                 return rc;
-        }
-        if ((rc = c->launch(KID_GRID2, [&] {
-                 k_grid2<DIM, R><<<nblocks(ni), kBlock, 0, s>>>(geom, gp, R(dt));
-             })) != MPM_OK)
-            return rc;
+            if ((rc = c->launch(KID_GRID1, [&] {
+                     k_grid1<DIM, R><<<gb, kBlock, 0, s>>>(geom, gp, sa, int(nc));
+                 })) != MPM_OK)
+                return rc;
+            // re-map gather (tiled: shared window, 256 threads); MUSL then scatters the
+            // new momentum (tiled: its own kernel; otherwise fused, global atomics)
+            if ((rc = c->launch(KID_G2P1, [&] {
+                     if (tiled)
+                         k_g2p1_tiled<DIM, S, TSH, R><<<sort.tiled_grid(), 256, 0, s>>>(
+                             geom, sort.tg, pp, gp, sa, sort.tile_list, sort.seg_begin,
+                             sort.seg_end, sort.tile_count);
+                     else
+                         k_g2p1_p2g2<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, sa, n, fl);
+                 })) != MPM_OK)
+                return rc;
+            if (tiled && sa.remap) {
+                if ((rc = c->launch(KID_P2G2, [&] {
+                         k_scatter_tiled<DIM, S, TSH, R, 1><<<sort.tiled_grid(), 64, 0, s>>>(
+                             geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
+                             sort.seg_begin, sort.seg_end, sort.tile_count, fl);
+                     })) != MPM_OK)
+                    return rc;
+            }
+            if (sa.remap) {
+                if ((rc = c->launch(KID_GRID2, [&] {
+                         k_grid2<DIM, R><<<nblocks(ni), kBlock, 0, s>>>(geom, gp, R(dt));
+                     })) != MPM_OK)
+                    return rc;
+            }
+            if (phase == PH_BODY) {
+                // USF: MVL after the position update, on the stress of this step's head
+                if (ar.mvl) {
+                    if ((rc = c->launch(KID_MVL1, [&] {
+                             k_mvl1<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, n);
+                         })) != MPM_OK)
+                        return rc;
+                    if ((rc = c->launch(KID_MVL2, [&] {
+                             k_mvl2<DIM, R><<<pb, kBlock, 0, s>>>(pp, gp, n);
+                         })) != MPM_OK)
+                        return rc;
+                }
+                return MPM_OK;
+            }
+        }  // phase != PH_HEAD
+        // the USF head's k_g2p2 leaves MVL to the body
+        StepArgs<R> sg = sa;
+        if (phase == PH_HEAD) sg.mvl = 0;
         auto g2p2 = [&](auto cm) {
             constexpr int CM = decltype(cm)::value;
             if (tiled)
                 k_g2p2_tiled<DIM, S, TSH, R, CM><<<sort.tiled_grid(), 256, 0, s>>>(
-                    geom, sort.tg, pp, gp, d_mats, sa, sort.tile_list, sort.seg_begin,
+                    geom, sort.tg, pp, gp, d_mats, sg, sort.tile_list, sort.seg_begin,
                     sort.seg_end, sort.tile_count, fl);
             else
-                k_g2p2<DIM, S, R, CM><<<pb, kBlock, 0, s>>>(geom, pp, gp, d_mats, sa, n, fl);
+                k_g2p2<DIM, S, R, CM><<<pb, kBlock, 0, s>>>(geom, pp, gp, d_mats, sg, n, fl);
         };
         if ((rc = c->launch(KID_G2P2, [&] {
                  if (ar.constitutive == MPM_DRUCKERPRAGER)
@@ -566,7 +608,7 @@ struct Impl : ImplBase {
                     return rc;
             }
         }
-        if (ar.mvl) {
+        if (ar.mvl && phase == PH_FULL) {
             if ((rc = c->launch(KID_MVL2, [&] {
                      k_mvl2<DIM, R><<<pb, kBlock, 0, s>>>(pp, gp, n);
                  })) != MPM_OK)
@@ -575,11 +617,12 @@ struct Impl : ImplBase {
         return MPM_OK;
     }
 
-    // the kernels of one step (no sort)
-    int kernels(mpm_ctx* c, const mpm_args& a, double dt, double Ti) {
+    // the kernels of one step, or of one part of a USF step (no sort)
+    int kernels(mpm_ctx* c, const mpm_args& a, double dt, double Ti, int phase) {
         const bool lin = c->cfg.basis == MPM_LINEAR;
-        if (dim == 2) return lin ? step_impl<2, 2>(c, a, dt, Ti) : step_impl<2, 3>(c, a, dt, Ti);
-        return lin ? step_impl<3, 2>(c, a, dt, Ti) : step_impl<3, 3>(c, a, dt, Ti);
+        if (dim == 2)
+            return lin ? step_impl<2, 2>(c, a, dt, Ti, phase) : step_impl<2, 3>(c, a, dt, Ti, phase);
+        return lin ? step_impl<3, 2>(c, a, dt, Ti, phase) : step_impl<3, 3>(c, a, dt, Ti, phase);
     }
 
     // the periodic cell sort, when due before step number c->steps
@@ -593,10 +636,18 @@ struct Impl : ImplBase {
         return MPM_OK;
     }
 
+    // One step.  USF: the head (stress update of the pending displacement and
+    // position commit) runs before the sort, so no increment is pending while
+    // the particles are permuted.
     int step(mpm_ctx* c, const mpm_args& a, double dt, double Ti) override {
-        int rc = sort_if_due(c);
-        if (rc == MPM_OK) rc = kernels(c, a, dt, Ti);
-        if (rc == MPM_OK) c->steps += 1;
+        const bool usf = a.scheme == MPM_USF;
+        int rc = usf ? kernels(c, a, dt, Ti, PH_HEAD) : MPM_OK;
+        if (rc == MPM_OK) rc = sort_if_due(c);
+        if (rc == MPM_OK) rc = kernels(c, a, dt, Ti, usf ? PH_BODY : PH_FULL);
+        if (rc == MPM_OK) {
+            c->steps += 1;
+            usf_pending = usf;
+        }
         return rc;
     }
 
@@ -604,18 +655,26 @@ struct Impl : ImplBase {
     // (Ti >= Te: full gravity, plasticity on) are captured once into a HIP
     // graph and replayed; sorts, ramp steps and runs of one step are eager.
     // The graph's pointers are those of the sort parity it was captured at.
+    // USF: each run starts with an eager head (before the sort); the graph
+    // holds body, then head + body for the following steps.  A context's
+    // first USF step is eager.
     int run_graph(mpm_ctx* c, const mpm_args& a, double dt, double Ti, int64_t nsteps,
                   double* Ti_out) override {
         const int K = c->cfg.sort_interval;
+        const bool usf = a.scheme == MPM_USF;
+        const int body = usf ? PH_BODY : PH_FULL;
         int64_t k = 0;
         int rc = MPM_OK;
         while (k < nsteps && rc == MPM_OK) {
+            const bool first_usf = usf && !usf_pending;
+            if (usf && (rc = kernels(c, a, dt, Ti, PH_HEAD)) != MPM_OK) break;
             if ((rc = sort_if_due(c)) != MPM_OK) break;
             long long L = nsteps - k;
             if (K > 0) L = std::min<long long>(L, K - (c->steps % K));
-            if (Ti < a.Te || L < 2 || c->prof_on) {
-                if ((rc = kernels(c, a, dt, Ti)) != MPM_OK) break;
+            if (Ti < a.Te || L < 2 || c->prof_on || first_usf) {
+                if ((rc = kernels(c, a, dt, Ti, body)) != MPM_OK) break;
                 c->steps += 1;
+                usf_pending = usf;
                 Ti += dt;
                 k += 1;
                 continue;
@@ -632,7 +691,8 @@ struct Impl : ImplBase {
                 if (e != hipSuccess) return c->hip_fail(e, "hipStreamBeginCapture");
                 double Tc = Ti;
                 for (long long j = 0; j < L && rc == MPM_OK; ++j) {
-                    rc = kernels(c, a, dt, Tc);
+                    if (usf && j > 0) rc = kernels(c, a, dt, Tc, PH_HEAD);
+                    if (rc == MPM_OK) rc = kernels(c, a, dt, Tc, body);
                     Tc += dt;
                 }
                 hipGraph_t graph = nullptr;
@@ -651,6 +711,7 @@ struct Impl : ImplBase {
             for (long long j = 0; j < L; ++j) Ti += dt;
             c->steps += L;
             k += L;
+            usf_pending = usf;
         }
         if (Ti_out) *Ti_out = Ti;
         return rc;
@@ -773,7 +834,19 @@ struct Impl : ImplBase {
                  double* ms) override {
         int rc;
         for (int a = 0; a < dim; ++a) {
-            if (xi && (rc = fetch(c, pp.x[a], xi + a * np, "download xi")) != MPM_OK) return rc;
+            if (xi && usf_pending) {
+                // a USF position increment is committed by the next step's head
+                k_unpermute_add<R><<<nblocks(np), kBlock, 0, c->stream>>>(pp.x[a], pp.dxp[a],
+                                                                          pp.pid, d_stage, int(np));
+                hipError_t e = hipGetLastError();
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(xi + a * np, d_stage, sizeof(double) * np,
+                                       hipMemcpyDeviceToHost, c->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+                if (e != hipSuccess) return c->hip_fail(e, "download xi");
+            } else if (xi && (rc = fetch(c, pp.x[a], xi + a * np, "download xi")) != MPM_OK) {
+                return rc;
+            }
             if (vs && (rc = fetch(c, pp.v[a], vs + a * np, "download vs")) != MPM_OK) return rc;
         }
         for (int k = 0; k < ns; ++k) {
@@ -911,7 +984,8 @@ static int mpm_step_check(mpm_ctx* c, const mpm_args* a, double dt) {
     if (!c->have_particles || !c->have_materials || !c->have_bc)
         return c->fail(MPM_E_STATE, "mpm_step: upload particles, materials and bc first");
     if (!(dt > 0.0) || !std::isfinite(dt)) return c->fail(MPM_E_ARG, "mpm_step: dt must be positive and finite");
-    if (a->scheme != MPM_MUSL) return c->fail(MPM_E_UNSUPPORTED, "mpm_step: only the MUSL scheme is implemented");
+    if (a->scheme != MPM_MUSL && a->scheme != MPM_USL && a->scheme != MPM_USF)
+        return c->fail(MPM_E_UNSUPPORTED, "mpm_step: scheme must be MUSL, USL or USF");
     if (a->va != MPM_VA_A && a->va != MPM_VA_V) return c->fail(MPM_E_ARG, "mpm_step: va must be :a or :v");
     if (a->constitutive < 0 || a->constitutive > 2)
         return c->fail(MPM_E_UNSUPPORTED, "mpm_step: constitutive must be linearelastic, druckerprager or mohrcoulomb");

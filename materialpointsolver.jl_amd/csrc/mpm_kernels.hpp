@@ -1,6 +1,6 @@
-// mpm_kernels.hpp -- gfx950 kernels of the explicit single-phase MUSL step.
+// mpm_kernels.hpp -- gfx950 kernels of the explicit single-phase step.
 //
-// One step is five launches (six with MVL):
+// A MUSL step is five launches (six with MVL):
 //   k_p2g        P2G #1: mass, momentum, internal + external force
 //                (P2G_OS!, src/solvers/utils_OS.jl:424-496, with the basis of
 //                resetmpstatus_OS!, utils_OS.jl:178-415, evaluated on the fly)
@@ -16,6 +16,11 @@
 //                liE!, dpP!, mcP!)
 //   k_mvl2       MVL second half (vollock2_OS!, utils_OS.jl:1202-1256); the
 //                first half (vollock1_OS!) is fused into k_g2p2
+//
+// USL (OS_USL.jl:102-144,325-366) drops the re-map: k_grid1 also writes the
+// nodal displacement and clears m, and k_g2p1_p2g2 / the tiled gather skip
+// the scatter.  USF (OS_USF.jl:11-97) is USL with k_g2p2 moved to the start
+// of the next step (k_mvl1 then does MVL's cell sums after the re-map).
 //
 // The basis is never stored: each particle pass recomputes the S per-axis
 // weights (S = 3 for uGIMP, 2 for linear) from the particle position, so a
@@ -125,6 +130,9 @@ template <typename R>
 struct StepArgs {
     R dt, G, flip, pic, zeta;
     int va;            // 0 = :a, 1 = :v
+    int gmode;         // re-map gather: 0 MUSL :a, 1 :v, 2 USL/USF :a (PIC on the new velocity)
+    int remap;         // 1: MUSL (momentum re-map); 0: USL/USF (grid solve #1 writes du)
+    int commit_dx;     // k_g2p2 adds the step's position increment (0: first USF step)
     int plastic;       // Ti >= Te
     int constitutive;  // 0 LE, 1 DP, 2 MC
     int mvl;
@@ -400,6 +408,7 @@ __global__ __launch_bounds__(256) void k_grid1(GridGeom g, GridPtrs<R> gp, StepA
         ss += fd * fd;
     }
     const R damp = -sa.zeta * sqrt(ss);
+    R vT[3];
     if (sa.va == 0) {
 #pragma unroll
         for (int d = 0; d < DIM; ++d)
@@ -408,14 +417,14 @@ __global__ __launch_bounds__(256) void k_grid1(GridGeom g, GridPtrs<R> gp, StepA
         for (int d = 0; d < DIM; ++d) {
             vs[d] = P[d] * msd;
             R Fs = Fc[d] + damp * sgn(vs[d]);
-            R vT = vs[d] + Fs * sa.dt * msd;
+            vT[d] = vs[d] + Fs * sa.dt * msd;
             R ac = Fs * msd;
             if (bc & (1u << d)) {
-                vT = gp.bcv[d][n];
+                vT[d] = gp.bcv[d][n];
                 ac = gp.bcv[d][n];
             }
             gp.vs[d][n] = vs[d];
-            gp.vsT[d][n] = vT;
+            gp.vsT[d][n] = vT[d];
             gp.as[d][n] = ac;
         }
     } else {
@@ -423,10 +432,10 @@ __global__ __launch_bounds__(256) void k_grid1(GridGeom g, GridPtrs<R> gp, StepA
         for (int d = 0; d < DIM; ++d) {
             vs[d] = P[d] * msd;
             R Fs = Fc[d] + damp * sgn(vs[d]);
-            R vT = (P[d] + Fs * sa.dt) * msd;
-            if (bc & (1u << d)) vT = gp.bcv[d][n];
+            vT[d] = (P[d] + Fs * sa.dt) * msd;
+            if (bc & (1u << d)) vT[d] = gp.bcv[d][n];
             gp.vs[d][n] = vs[d];
-            gp.vsT[d][n] = vT;
+            gp.vsT[d][n] = vT[d];
         }
     }
 #pragma unroll
@@ -434,22 +443,31 @@ __global__ __launch_bounds__(256) void k_grid1(GridGeom g, GridPtrs<R> gp, StepA
         gp.p[d][n] = R(0);
         gp.f[d][n] = R(0);
     }
+    if (!sa.remap) {
+        // USL/USF: Δus = v_new Δt (solvegrid(_a)_USL_OS!, OS_USL.jl:63-100,225-269);
+        // nothing else reads this node's mass in the step
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) gp.du[d][n] = vT[d] * sa.dt;
+        gp.m[n] = R(0);
+    }
 }
 
 // ---------------------------------------------------------------------------
 // MUSL re-map (doublemapping1(_a)_OS! + doublemapping2_OS!, utils_OS.jl:698-959)
 // ---------------------------------------------------------------------------
 // One node's share of the re-map sums: A = Σ N vsT, and for :a B = Σ N as,
-// C = Σ N vs; for :v B = Σ N (vsT - vs), C = Σ N vsT.
+// C = Σ N vs (MUSL, gmode 0) or Σ N vsT (USL/USF, gmode 2: the reference's
+// doublemapping1_a_USL_OS! gathers the already updated nodal velocity,
+// OS_USL.jl:271-313); for :v (gmode 1) B = Σ N (vsT - vs), C = Σ N vsT.
 template <int DIM, typename R>
-__device__ __forceinline__ void g2p1_accumulate(int va, R N, const R* vT, const R* vA, const R* vS,
-                                                R* A, R* B, R* C) {
+__device__ __forceinline__ void g2p1_accumulate(int gmode, R N, const R* vT, const R* vA,
+                                                const R* vS, R* A, R* B, R* C) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        if (va == 0) {
+        if (gmode != 1) {
             A[d] += N * vT[d];
             B[d] += N * vA[d];
-            C[d] += N * vS[d];
+            C[d] += N * (gmode == 2 ? vT[d] : vS[d]);
         } else {
             A[d] += N * vT[d];
             B[d] += N * (vT[d] - vS[d]);
@@ -466,8 +484,8 @@ __device__ __forceinline__ void g2p1_commit(PartPtrs<R>& pp, const StepArgs<R>& 
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         R v = pp.v[d][i];
-        R vn = (sa.va == 0) ? sa.flip * v + sa.pic * C[d] + B[d] * sa.dt
-                            : sa.flip * (v + B[d]) + sa.pic * C[d];
+        R vn = (sa.gmode != 1) ? sa.flip * v + sa.pic * C[d] + B[d] * sa.dt
+                               : sa.flip * (v + B[d]) + sa.pic * C[d];
         pp.v[d][i] = vn;
         pp.dxp[d][i] = A[d] * sa.dt;
         pv[d] = m * vn;
@@ -498,11 +516,12 @@ __global__ __launch_bounds__(256) void k_g2p1_p2g2(GridGeom g, PartPtrs<R> pp, G
                 vA[d] = gp.as[d][n];
                 vS[d] = gp.vs[d][n];
             }
-            g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
+            g2p1_accumulate<DIM, R>(sa.gmode, N, vT, vA, vS, A, B, C);
         }
     });
     R pv[3];
     g2p1_commit<DIM, R>(pp, sa, i, A, B, C, pv);
+    if (!sa.remap) return;  // USL/USF: no momentum re-map
     for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
         if (N != R(0)) {
 #pragma unroll
@@ -950,8 +969,10 @@ __device__ __forceinline__ void g2p2_commit(const GridGeom& g, PartPtrs<R>& pp, 
         atomic_add(&gp.cell_sm[c], vol * sm);
         atomic_add(&gp.cell_vol[c], vol);
     }
+    if (sa.commit_dx) {
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) pp.x[d][i] = x[d] + double(pp.dxp[d][i]);
+        for (int d = 0; d < DIM; ++d) pp.x[d][i] = x[d] + double(pp.dxp[d][i]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -995,6 +1016,32 @@ __global__ __launch_bounds__(256) void k_g2p2(GridGeom g, PartPtrs<R> pp, GridPt
     });
     g2p2_commit<DIM, R, CM>(g, pp, gp, mats, sa, i, x, F);
     raise_flag(flags, flag);
+}
+
+// vollock1_OS! (utils_OS.jl:1162-1193) on its own, for USF, whose MVL runs
+// after the position update (OS_USF.jl:49-52,92-95): cell sums of Ω σm and Ω
+// by the step-start cell (x is not committed until the next step).
+template <int DIM, int S, typename R>
+__global__ __launch_bounds__(256) void k_mvl1(GridGeom g, PartPtrs<R> pp, GridPtrs<R> gp, int np) {
+    constexpr int NS = DIM == 2 ? 4 : 6;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    double x[3];
+    load_x<DIM>(pp.x, i, x);
+    Stencil<DIM, S, R> st;
+    int flag = 0;
+    if (!make_stencil<DIM, S, R>(g, x, st, flag)) {
+        pp.p2c[i] = -1;  // frozen particle: no cell
+        return;
+    }
+    R s[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s[k] = pp.sig[k][i];
+    const int c = cell_of<DIM>(g, x);
+    pp.p2c[i] = c;
+    const R vol = pp.vol[i];
+    atomic_add(&gp.cell_sm[c], vol * mean3<NS>(s));
+    atomic_add(&gp.cell_vol[c], vol);
 }
 
 // vollock2_OS! (utils_OS.jl:1202-1256)

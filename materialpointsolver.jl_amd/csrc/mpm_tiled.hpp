@@ -458,7 +458,7 @@ __global__ __launch_bounds__(256) void k_g2p1_tiled(GridGeom g, TileGeom tg, Par
                         vA[d] = L[(DIM + d) * WN];
                         vS[d] = L[(2 * DIM + d) * WN];
                     }
-                    g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
+                    g2p1_accumulate<DIM, R>(sa.gmode, N, vT, vA, vS, A, B, C);
                 });
             } else {
                 for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(256) void k_g2p1_tiled(GridGeom g, TileGeom tg, Par
                             vA[d] = gp.as[d][n];
                             vS[d] = gp.vs[d][n];
                         }
-                        g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
+                        g2p1_accumulate<DIM, R>(sa.gmode, N, vT, vA, vS, A, B, C);
                     }
                 });
             }

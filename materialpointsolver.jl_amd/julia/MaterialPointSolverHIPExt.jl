@@ -9,7 +9,9 @@
   What it adds (all by multiple dispatch, nothing in the package is edited):
     host2device(grid, mp, attr, bc, Val(:HIP))      -> HIP handles (mpm_create + uploads)
     procedure!(args, ::HIPGrid, ::HIPParticle, ::HIPProperty, ::HIPVBoundary,
-               ΔT, Ti, Val(:OS), Val(:MUSL), Val(va))  -> one mpm_step
+               ΔT, Ti, Val(:OS), Val(scheme), Val(va)) -> one mpm_step
+               (scheme :MUSL | :USL | :USF, va :a | :v; one method per pair, so each
+               is strictly more specific than the package's own methods)
     device2host!(args, mp, ::HIPParticle, Val(:HIP)) -> mpm_download_particles
     clean_device!(::HIPGrid, ..., Val(:HIP))          -> mpm_destroy
     getBackend(Val(:HIP))                             -> KernelAbstractions.CPU()
@@ -52,6 +54,7 @@ extra_gravity(ext, name) = hasproperty(ext, name) ? Float64(getproperty(ext, nam
 
 const CONSTITUTIVE = Dict(:linearelastic => Int32(0), :druckerprager => Int32(1),
                           :mohrcoulomb => Int32(2))
+const SCHEME = Dict(:MUSL => Int32(0), :USL => Int32(1), :USF => Int32(2))
 
 # ---- device-side handles: the abstract supertypes the package dispatches on -------------
 mutable struct HIPContext
@@ -120,12 +123,8 @@ function host2device(
             HIPProperty{T1, T2}(ctx), HIPVBoundary{T1, T2}(ctx))
 end
 
-function procedure!(
-    args::DeviceArgs{T1, T2}, grid::HIPGrid{T1, T2}, mp::HIPParticle{T1, T2},
-    attr::HIPProperty{T1, T2}, bc::HIPVBoundary{T1, T2}, ΔT::T2, Ti::T2,
-    ::Val{:OS}, ::Val{:MUSL}, ::Val{va}
-) where {T1, T2, va}
-    a = MPMArgs(Int32(0), va == :a ? Int32(0) : Int32(1), CONSTITUTIVE[args.constitutive],
+function hip_step!(args, grid::HIPGrid, ΔT, Ti, scheme::Symbol, va::Symbol)
+    a = MPMArgs(SCHEME[scheme], va == :a ? Int32(0) : Int32(1), CONSTITUTIVE[args.constitutive],
         Int32(args.MVL), Float64(args.FLIP), Float64(args.PIC), Float64(args.ζs),
         Float64(args.gravity), Float64(args.Te), extra_gravity(args.ext, :gravity_x),
         extra_gravity(args.ext, :gravity_y))
@@ -133,6 +132,17 @@ function procedure!(
     check(p, ccall((:mpm_step, LIB), Cint, (Ptr{Cvoid}, Ref{MPMArgs}, Cdouble, Cdouble),
         p, a, Float64(ΔT), Float64(Ti)))
     return nothing
+end
+
+# OS_MUSL.jl:12-102, OS_USL.jl:102-144,325-366, OS_USF.jl:11-97
+for S in (:MUSL, :USL, :USF), VA in (:a, :v)
+    @eval function procedure!(
+        args::DeviceArgs{T1, T2}, grid::HIPGrid{T1, T2}, mp::HIPParticle{T1, T2},
+        attr::HIPProperty{T1, T2}, bc::HIPVBoundary{T1, T2}, ΔT::T2, Ti::T2,
+        ::Val{:OS}, ::Val{$(QuoteNode(S))}, ::Val{$(QuoteNode(VA))}
+    ) where {T1, T2}
+        return hip_step!(args, grid, ΔT, Ti, $(QuoteNode(S)), $(QuoteNode(VA)))
+    end
 end
 
 function device2host!(

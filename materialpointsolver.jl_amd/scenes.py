@@ -45,14 +45,14 @@ def column_collapse_2d(*, h=0.0025, width=0.2, height=0.1, box_x=0.5, box_y=0.12
                        dt=2e-5, Ttol=None, nsteps=50_000, eps="FP64", constitutive="druckerprager",
                        phi_deg=PHI_DEG, cohesion=0.0, psi_deg=0.0, sigma_t=0.0, Hp=0.0, cr=0.0,
                        va="a", MVL=False, zeta=0.0, Te=0.0, gravity=-9.8, basis="uGIMP",
-                       sort_interval=None, device="HIP", project_name="collapse2d",
+                       scheme="MUSL", sort_interval=None, device="HIP", project_name="collapse2d",
                        project_path="."):
     """Plane-strain granular column collapse; returns (args, grid, mp, attr, bc)."""
     if Ttol is None:
         Ttol = dt * nsteps
     nic = 9 if basis == "uGIMP" else 4
     args = UserArgs2D(Ttol=Ttol, Te=Te, ΔT=dt, FLIP=1, PIC=0, constitutive=constitutive,
-                      basis=basis, MVL=MVL, device=device, coupling="OS", scheme="MUSL",
+                      basis=basis, MVL=MVL, device=device, coupling="OS", scheme=scheme,
                       va=va, gravity=gravity, ζs=zeta, project_name=project_name,
                       project_path=project_path, ϵ=eps, progressbar=False,
                       sort_interval=sort_interval)
@@ -76,7 +76,7 @@ def column_collapse_3d(*, h=0.005, width=0.4, height=0.2, box_xy=1.2, box_z=0.25
                        dt=None, cfl=0.2, Ttol=None, nsteps=20_000, eps="FP32",
                        constitutive="druckerprager", phi_deg=PHI_DEG, cohesion=0.0, psi_deg=0.0,
                        sigma_t=0.0, va="a", MVL=False, zeta=0.0, Te=0.0, gravity=-9.8,
-                       basis="uGIMP", sort_interval=None, device="HIP",
+                       basis="uGIMP", scheme="MUSL", sort_interval=None, device="HIP",
                        project_name="collapse3d", project_path="."):
     """3D granular column collapse centred in x and y; returns (args, grid, mp, attr, bc).
     Floor z <= 0 no-slip; walls at x, y = +-box_xy/2 slip (normal velocity fixed)."""
@@ -87,7 +87,7 @@ def column_collapse_3d(*, h=0.005, width=0.4, height=0.2, box_xy=1.2, box_z=0.25
     nic = 27 if basis == "uGIMP" else 8
     half = box_xy / 2
     args = UserArgs3D(Ttol=Ttol, Te=Te, ΔT=dt, FLIP=1, PIC=0, constitutive=constitutive,
-                      basis=basis, MVL=MVL, device=device, coupling="OS", scheme="MUSL",
+                      basis=basis, MVL=MVL, device=device, coupling="OS", scheme=scheme,
                       va=va, gravity=gravity, ζs=zeta, project_name=project_name,
                       project_path=project_path, ϵ=eps, progressbar=False,
                       sort_interval=sort_interval)

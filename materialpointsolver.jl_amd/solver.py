@@ -67,21 +67,67 @@ class HIPModel:
     def _check(self, rc):
         return capi.check(self.lib, self.ctx, rc)
 
-    def _upload(self, grid, mp, attr, bc):
-        f64 = lambda a: np.asfortranarray(np.asarray(a, dtype=np.float64))
-        i64 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int64))
-        n = self.np
-        arrs = dict(
-            xi=f64(mp.ξ), vs=f64(mp.vs), sigma=f64(mp.σij), eps_s=f64(mp.ϵijs), F=f64(mp.F),
-            omega=f64(mp.Ω), omega0=f64(mp.Ω0), rho=f64(mp.ρs), rho0=f64(mp.ρs0),
-            eps_q=f64(mp.ϵq), eps_k=f64(mp.ϵk), nid=i64(attr.nid))
-        if arrs["nid"].size != n:
+    def _upload_particles(self, arrs):
+        if arrs["nid"].size != self.np:
             raise ValueError("attr.nid must have one entry per particle")
+        for k in ("xi", "vs", "sigma", "eps_s", "F", "omega", "omega0", "rho", "rho0", "eps_q",
+                  "eps_k"):
+            if arrs[k].shape[0] != self.np:
+                raise ValueError(f"{k} must have np = {self.np} rows")
         d = capi.dptr
         self._check(self.lib.mpm_upload_particles(
             self.ctx, d(arrs["xi"]), d(arrs["vs"]), d(arrs["sigma"]), d(arrs["eps_s"]),
             d(arrs["F"]), d(arrs["omega"]), d(arrs["omega0"]), d(arrs["rho"]), d(arrs["rho0"]),
             d(arrs["eps_q"]), d(arrs["eps_k"]), capi.iptr(arrs["nid"])))
+
+    @staticmethod
+    def snapshot_state(group, Ω0, nid):
+        """Particle arrays for mpm_upload_particles from one snapshot group (the
+        datasets of src/solver.jl:60-83: coords, velocity_s, stress, strain_s,
+        eqstrain, ekstrain, volume, mass_s, time).  The reference saves neither
+        F nor ρs: det F is re-derived as volume / Ω0 (the step only reads det F)
+        and the densities as mass_s / volume and mass_s / Ω0.  Returns
+        (arrays, time)."""
+        g = group if isinstance(group, dict) else dict(np.load(group))
+        f64 = lambda a: np.asfortranarray(np.asarray(a, dtype=np.float64))
+        x = f64(g["coords"])
+        n, dim = x.shape
+        vol = np.asarray(g["volume"], dtype=np.float64).reshape(n)
+        vol0 = np.asarray(Ω0, dtype=np.float64).reshape(n)
+        ms = np.asarray(g["mass_s"], dtype=np.float64).reshape(n)
+        F = np.zeros((n, dim * dim))
+        F[:, 0] = vol / vol0                 # det F = J, row-major diag(J, 1[, 1])
+        F[:, dim + 1] = 1.0
+        if dim == 3:
+            F[:, 8] = 1.0
+        arrs = dict(xi=x, vs=f64(g["velocity_s"]), sigma=f64(g["stress"]),
+                    eps_s=f64(g["strain_s"]), F=f64(F), omega=f64(vol), omega0=f64(vol0),
+                    rho=f64(ms / vol), rho0=f64(ms / vol0), eps_q=f64(g["eqstrain"]),
+                    eps_k=f64(g["ekstrain"]),
+                    nid=np.ascontiguousarray(np.asarray(nid, dtype=np.int64)))
+        return arrs, float(np.asarray(g["time"]))
+
+    def upload_snapshot(self, group, Ω0, nid=None):
+        """Resume from a snapshot group written by SnapshotWriter (the reference's
+        HDF5 group layout): re-uploads the particle state and returns the
+        group's time, the Ti to continue from.  Ω0 is the particles' initial
+        volume (mp.Ω0 of the host struct; the reference's snapshot does not hold
+        it), nid defaults to the material ids uploaded at creation.  A USF run's
+        pending displacement is not part of a snapshot (nor of the
+        reference's): the next USF step starts from a zero increment."""
+        arrs, Ti = self.snapshot_state(group, Ω0, self._nid if nid is None else nid)
+        self._upload_particles(arrs)
+        return Ti
+
+    def _upload(self, grid, mp, attr, bc):
+        f64 = lambda a: np.asfortranarray(np.asarray(a, dtype=np.float64))
+        i64 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+        arrs = dict(
+            xi=f64(mp.ξ), vs=f64(mp.vs), sigma=f64(mp.σij), eps_s=f64(mp.ϵijs), F=f64(mp.F),
+            omega=f64(mp.Ω), omega0=f64(mp.Ω0), rho=f64(mp.ρs), rho0=f64(mp.ρs0),
+            eps_q=f64(mp.ϵq), eps_k=f64(mp.ϵk), nid=i64(attr.nid))
+        self._upload_particles(arrs)
+        self._nid = arrs["nid"]
         m = [f64(a) for a in (attr.Ks, attr.Gs, attr.ϕ, attr.ψ, attr.c, attr.σt, attr.Hp, attr.cr)]
         self._check(self.lib.mpm_upload_materials(self.ctx, *[capi.dptr(a) for a in m]))
         ni = int(grid.ni)
@@ -100,11 +146,12 @@ class HIPModel:
     @staticmethod
     def step_args(args):
         a = capi.MPMArgs()
-        if args.scheme != "MUSL" or args.coupling != "OS":
-            raise NotImplementedError("the HIP path implements coupling=:OS, scheme=:MUSL")
+        if args.coupling != "OS" or args.scheme not in capi.SCHEME:
+            raise NotImplementedError("the HIP path implements coupling=:OS with the "
+                                      "MUSL, USL and USF schemes")
         if args.constitutive not in capi.CONSTITUTIVE:
             raise NotImplementedError(f"constitutive {args.constitutive} has no HIP kernels")
-        a.scheme = capi.MPM_MUSL
+        a.scheme = capi.SCHEME[args.scheme]
         a.va = capi.MPM_VA_A if args.va == "a" else capi.MPM_VA_V
         a.constitutive = capi.CONSTITUTIVE[args.constitutive]
         a.mvl = 1 if args.MVL else 0
@@ -270,13 +317,16 @@ def KAsync(dev_handle):
 
 
 def procedure(args, grid, mp, attr, bc, ΔT, Ti, coupling, scheme, va):
-    """procedure!(args, grid, mp, attr, bc, ΔT, Ti, Val(:OS), Val(:MUSL), Val(va))
-    -- src/solvers/OS_MUSL.jl:12-102, as one device step."""
-    if coupling != "OS" or scheme != "MUSL":
-        raise NotImplementedError("the HIP path implements Val(:OS), Val(:MUSL)")
+    """procedure!(args, grid, mp, attr, bc, ΔT, Ti, Val(:OS), Val(scheme), Val(va))
+    -- src/solvers/OS_MUSL.jl:12-102, OS_USL.jl:102-144,325-366 and
+    OS_USF.jl:11-97, as one device step."""
+    if coupling != "OS" or scheme not in capi.SCHEME:
+        raise NotImplementedError("the HIP path implements Val(:OS) with Val(:MUSL), "
+                                  "Val(:USL) and Val(:USF)")
     if not isinstance(mp, _DeviceHandle):
         raise TypeError("procedure needs the device structs returned by host2device")
     sargs = HIPModel.step_args(args)
+    sargs.scheme = capi.SCHEME[scheme]
     sargs.va = 0 if va == "a" else 1
     mp.model.step(sargs, ΔT, Ti)
 

@@ -1,6 +1,7 @@
 /*
  * mpm_oracle.c -- FP64 CPU restatement of MaterialPointSolver.jl's single-phase
- * explicit MPM step (OS coupling, MUSL scheme).  TEST INFRASTRUCTURE ONLY.
+ * explicit MPM step (OS coupling; MUSL, USL and USF schemes).  TEST
+ * INFRASTRUCTURE ONLY.
  *
  * This file is the parity checker for the HIP library.  Only tests, the smoke
  * check and the CPU-baseline leg of bench.py may load it; the product path
@@ -67,6 +68,8 @@ typedef struct oracle_model {
     /* ---- extension: body force on the other axes (0 = the reference) ---- */
     double gravity_x, gravity_y;
     double Ti_now;         /* time of the current step (set by oracle_procedure) */
+    int32_t scheme;        /* 0 = :MUSL, 1 = :USL, 2 = :USF */
+    int32_t pad1;
 } oracle_model;
 
 #define COL(a, n, i, k) ((a)[(int64_t)(k) * (n) + (i)])   /* (i,k) 0-based of (n, .) */
@@ -375,6 +378,11 @@ static inline int64_t bc_idx(const int64_t *a, int64_t i) { return a[i]; }
 /* ---------------------------------------------------------------------------
  * solvegrid_a_OS! (utils_OS.jl:547-583 2D, 595-640 3D)
  * solvegrid_OS!   (utils_OS.jl:507-535 2D, 650-687 3D)
+ * USL / USF: solvegrid_a_USL_OS! (OS_USL.jl:147-187 2D, 225-269 3D) and
+ * solvegrid_USL_OS! (OS_USL.jl:23-56 2D, 63-100 3D) are the same arithmetic
+ * plus the nodal displacement dus = v_new * dt, and leave ps alone.  In the :a
+ * form the reference stores the new velocity in grid.vs; it is kept in g_vsT
+ * here (the pre-update velocity is not read again in those schemes).
  * ------------------------------------------------------------------------- */
 void oracle_solvegrid(oracle_model *m, double dt) {
     const int64_t ni = m->ni, D = m->dim;
@@ -414,13 +422,20 @@ void oracle_solvegrid(oracle_model *m, double dt) {
             for (int64_t d = 0; d < D; ++d)
                 if (bc_idx(idx[d], i) != 0) COL(m->g_vsT, ni, i, d) = val[d][i];
         }
-        for (int64_t d = 0; d < D; ++d) COL(m->g_ps, ni, i, d) = 0.0;
+        if (m->scheme == 0) {
+            for (int64_t d = 0; d < D; ++d) COL(m->g_ps, ni, i, d) = 0.0;
+        } else {
+            for (int64_t d = 0; d < D; ++d) COL(m->g_dus, ni, i, d) = COL(m->g_vsT, ni, i, d) * dt;
+        }
     }
 }
 
 /* ---------------------------------------------------------------------------
  * doublemapping1_a_OS! (utils_OS.jl:745-788 2D, 856-908 3D)
  * doublemapping1_OS!   (utils_OS.jl:698-743 2D, 799-854 3D)
+ * doublemapping1_a_USL_OS! (OS_USL.jl:189-223 2D, 271-313 3D): as the :a form
+ * but the PIC term gathers the NEW nodal velocity (the USL grid solve
+ * overwrote grid.vs with it).  USL/USF :v use doublemapping1_OS! unchanged.
  * The reference loads Ks/Gs there without using them; nothing to restate.
  * ------------------------------------------------------------------------- */
 void oracle_dm1(oracle_model *m, double dt) {
@@ -437,7 +452,7 @@ void oracle_dm1(oracle_model *m, double dt) {
                     if (m->va == 0) {
                         a[d] += Ni * vT;                          /* tmp_p */
                         b[d] += Ni * COL(m->g_as, ni, n, d);      /* tmp_v */
-                        c[d] += Ni * COL(m->g_vs, ni, n, d);      /* tmp_T */
+                        c[d] += Ni * (m->scheme == 0 ? COL(m->g_vs, ni, n, d) : vT);
                     } else {
                         a[d] += Ni * vT;                          /* tmp_pos */
                         b[d] += Ni * (vT - COL(m->g_vs, ni, n, d)); /* tmp_s1 */
@@ -957,22 +972,9 @@ void oracle_vollock(oracle_model *m) {
     }
 }
 
-/* ---------------------------------------------------------------------------
- * procedure!(..., Val(:OS), Val(:MUSL), Val(:a|:v)) (src/solvers/OS_MUSL.jl:12-102)
- * ------------------------------------------------------------------------- */
-void oracle_procedure(oracle_model *m, double dt, double Ti) {
-#ifdef _OPENMP
-    if (m->nthreads > 1) omp_set_num_threads(m->nthreads);
-#endif
-    double G = Ti < m->Te ? m->gravity / m->Te * Ti : m->gravity;
-    m->Ti_now = Ti;
-    oracle_resetgrid(m);
-    oracle_resetmp(m);
-    oracle_p2g(m, G);
-    oracle_solvegrid(m, dt);
-    oracle_dm1(m, dt);
-    oracle_dm2(m);
-    oracle_dm3(m, dt);
+/* G2P_OS! and the constitutive update of one step (the same block in
+ * OS_MUSL.jl:85-96, OS_USL.jl:123-138,346-361 and OS_USF.jl:25-40,68-83). */
+static void g2p_and_stress(oracle_model *m, double dt, double Ti) {
     oracle_g2p(m, dt);
     if (m->constitutive == 0) {
         oracle_liE(m);
@@ -983,6 +985,33 @@ void oracle_procedure(oracle_model *m, double dt, double Ti) {
         oracle_liE(m);
         if (Ti >= m->Te) oracle_mcP(m);
     }
+}
+
+/* ---------------------------------------------------------------------------
+ * procedure!(..., Val(:OS), Val(:MUSL), Val(:a|:v)) (src/solvers/OS_MUSL.jl:12-102)
+ * procedure!(..., Val(:OS), Val(:USL), Val(:a|:v))  (src/solvers/OS_USL.jl:102-144,325-366)
+ * procedure!(..., Val(:OS), Val(:USF), Val(:a|:v))  (src/solvers/OS_USF.jl:11-54,56-97)
+ * USF updates the stress FIRST, with the nodal displacement and the basis of
+ * the previous step (zero displacement and the initial basis at the first
+ * step), and runs MVL after the position update.
+ * ------------------------------------------------------------------------- */
+void oracle_procedure(oracle_model *m, double dt, double Ti) {
+#ifdef _OPENMP
+    if (m->nthreads > 1) omp_set_num_threads(m->nthreads);
+#endif
+    double G = Ti < m->Te ? m->gravity / m->Te * Ti : m->gravity;
+    m->Ti_now = Ti;
+    if (m->scheme == 2) g2p_and_stress(m, dt, Ti);
+    oracle_resetgrid(m);
+    oracle_resetmp(m);
+    oracle_p2g(m, G);
+    oracle_solvegrid(m, dt);
+    oracle_dm1(m, dt);
+    if (m->scheme == 0) {
+        oracle_dm2(m);
+        oracle_dm3(m, dt);
+    }
+    if (m->scheme != 2) g2p_and_stress(m, dt, Ti);
     if (m->mvl) oracle_vollock(m);
 }
 

@@ -78,6 +78,7 @@ class OracleModel(C.Structure):
         ("vx_idx", _I), ("vy_idx", _I), ("vz_idx", _I), ("vx_val", _D), ("vy_val", _D),
         ("vz_val", _D),
         ("gravity_x", C.c_double), ("gravity_y", C.c_double), ("Ti_now", C.c_double),
+        ("scheme", C.c_int32), ("pad1", C.c_int32),
     ]
 
 
@@ -212,6 +213,7 @@ class Oracle:
         m.dim, m.nic = D, nic
         m.basis = 1 if grid.NIC in (4, 8) else 0
         m.va = 0 if args.va == "a" else 1
+        m.scheme = {"MUSL": 0, "USL": 1, "USF": 2}[args.scheme]
         m.constitutive = _CONST[args.constitutive]
         m.mvl = 1 if args.MVL else 0
         m.nthreads = int(nthreads)
@@ -238,6 +240,9 @@ class Oracle:
             else:
                 setattr(m, k, v.ctypes.data_as(_D))
         self._p = C.byref(m)
+        # initmpstatus! before the loop (src/solver.jl:45): the basis that the
+        # first USF step's G2P reads
+        self.initmp()
 
     # whole step / loop
     def procedure(self, dt, Ti):

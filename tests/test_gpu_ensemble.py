@@ -66,6 +66,15 @@ def test_graph_run_unsorted_and_mc3d(mpm):
     _graph_vs_eager(mpm, sc, [10, 10, 5], sort_interval=0)
 
 
+@pytest.mark.parametrize("scheme", ["USL", "USF"])
+def test_graph_run_matches_eager_schemes(mpm, scheme):
+    # USF graphs hold body, then head + body; each run of steps starts with an
+    # eager head before the sort
+    sc = mpm.scenes.column_collapse_3d(h=0.02, width=0.16, height=0.08, box_xy=0.4, box_z=0.12,
+                                       nsteps=100, eps="FP64", scheme=scheme, MVL=True)
+    _graph_vs_eager(mpm, sc, [1, 9, 12, 3, 20], sort_interval=5)
+
+
 def test_packed_instances_evolve_independently(mpm):
     kw = dict(n_instances=5, seed=11, np_range=(400, 2500), row_width=1.0, eps="FP64")
     args, grid, mp, attr, bc, lay = mpm.scenes.ensemble_2d(**kw)
@@ -139,3 +148,42 @@ def test_autotune_on_the_device(mpm):
         assert np.all(np.isfinite(st["xi"]))
     finally:
         m.close()
+
+
+@pytest.mark.parametrize("eps", ["FP64", "FP32"])
+def test_resume_from_snapshot(mpm, tmp_path, eps):
+    # run 40 steps, write a snapshot group, run 40 more; a new context resumed
+    # from the group and run 40 steps lands on the same state (FP64: to atomic
+    # summation order; FP32: at the tolerance of its arithmetic)
+    args, grid, mp, attr, bc = mpm.scenes.column_collapse_3d(
+        h=0.02, width=0.16, height=0.08, box_xy=0.4, box_z=0.12, nsteps=100, eps=eps,
+        project_name="resume", project_path=str(tmp_path))
+    dt = float(args.ΔT)
+    a = mpm.HIPModel(grid, mp, attr, bc, sort_interval=5)
+    try:
+        sa = a.step_args(args)
+        Ti = a.run(sa, dt, 0.0, 40)
+        snap = a.download()
+        w = mpm.solver.SnapshotWriter(args)
+        host = type("MP", (), {})()
+        host.σij, host.ϵijs, host.ϵq, host.ϵk = snap["sigma"], snap["eps_s"], snap["eps_q"], snap["eps_k"]
+        host.ϵv, host.ξ, host.vs, host.Ω, host.ms = snap["eps_v"], snap["xi"], snap["vs"], snap["omega"], snap["ms"]
+        w.group(1, host, Ti)
+        a.run(sa, dt, Ti, 40)
+        ref = a.download()
+    finally:
+        a.close()
+    b = mpm.HIPModel(grid, mp, attr, bc, sort_interval=5)
+    try:
+        T0 = b.upload_snapshot(str(tmp_path / "resume" / "resume_snapshots" / "group1.npz"), mp.Ω0)
+        assert T0 == Ti
+        b.run(b.step_args(args), dt, T0, 40)
+        got = b.download()
+    finally:
+        b.close()
+    tol = 1e-9 if eps == "FP64" else 2e-4
+    x0 = np.asarray(mp.ξ, dtype=np.float64)
+    for k in ("xi", "vs", "sigma", "eps_q", "omega"):
+        r = _rel(got[k] - (x0 if k == "xi" else 0), ref[k] - (x0 if k == "xi" else 0))
+        print(f"  {k}: rel L2 {r:.1e}", flush=True)
+        assert r <= tol, k

@@ -85,6 +85,49 @@ def test_fp64_3d_mohrcoulomb(mpm, orc, va, sort):
              100, 1e-9, sort_interval=sort)
 
 
+# USL and USF schedules (OS_USL.jl, OS_USF.jl); PIC = 0.3 so that the USL :a
+# re-map's PIC term (gathered from the new nodal velocity) is exercised
+@pytest.mark.parametrize("sort", [0, 5])
+@pytest.mark.parametrize("va", ["a", "v"])
+@pytest.mark.parametrize("scheme", ["USL", "USF"])
+def test_fp64_2d_schemes(mpm, orc, scheme, va, sort):
+    sc = _c2(mpm, constitutive="druckerprager", va=va, cohesion=200.0, scheme=scheme)
+    sc[0].FLIP, sc[0].PIC = 0.7, 0.3
+    _compare(mpm, orc, sc, 200, 1e-9, sort_interval=sort)
+
+
+@pytest.mark.parametrize("sort", [0, 5])
+@pytest.mark.parametrize("constitutive", ["linearelastic", "mohrcoulomb"])
+@pytest.mark.parametrize("scheme", ["USL", "USF"])
+def test_fp64_3d_schemes(mpm, orc, scheme, constitutive, sort):
+    sc = _c3(mpm, constitutive=constitutive, cohesion=200.0, psi_deg=5.0, scheme=scheme)
+    sc[0].FLIP, sc[0].PIC = 0.7, 0.3
+    _compare(mpm, orc, sc, 100, 1e-9, sort_interval=sort)
+
+
+def test_fp64_schemes_mvl_and_ramp(mpm, orc):
+    # USF runs MVL after the position update, on the stress of the step's head
+    for scheme in ("USL", "USF"):
+        _compare(mpm, orc, _c2(mpm, MVL=True, scheme=scheme, Te=0.004), 200, 1e-9)
+        _compare(mpm, orc, _c3(mpm, MVL=True, scheme=scheme), 60, 1e-9, sort_interval=4)
+
+
+@pytest.mark.parametrize("scheme", ["USL", "USF"])
+def test_fp32_3d_usl_usf(mpm, orc, scheme):
+    # USL/USF have no velocity re-map: the strain increment comes from
+    # (p + f dt) / m, which is large at nodes that hold a sliver of mass, so the
+    # step is far worse conditioned than MUSL's.  In the FP64 oracle a relative
+    # perturbation of 6e-8 (FP32 rounding) of the initial positions of this
+    # column changes σ after 100 steps by 1.0-1.4e-2 relative L2 (Drucker-Prager,
+    # c = 0) and 8-9e-4 (linear elastic), against 7e-7 and 3.5e-7 under MUSL.
+    # The FP32 bars below are about twice that conditioning.
+    tol = dict(xi=2e-4, vs=1e-3, sigma=3e-2, eps_s=5e-4, eps_q=3e-2, omega=1e-5)
+    _compare(mpm, orc, _c3(mpm, eps="FP32", scheme=scheme), 100, tol, sort_interval=10)
+    tol = dict(xi=2e-4, vs=5e-4, sigma=3e-3, eps_s=5e-4, eps_q=3e-3, omega=1e-5)
+    _compare(mpm, orc, _c3(mpm, eps="FP32", scheme=scheme, constitutive="linearelastic"), 100,
+             tol, sort_interval=10)
+
+
 def test_fp32_3d_mohrcoulomb(mpm, orc):
     tol = dict(xi=2e-4, vs=5e-4, sigma=2e-3, eps_s=5e-4, eps_q=2e-3, omega=1e-5)
     _compare(mpm, orc, _c3(mpm, eps="FP32", constitutive="mohrcoulomb", cohesion=200.0,

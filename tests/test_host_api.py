@@ -64,6 +64,19 @@ def test_particle_and_property(mpm):
     assert at.nmat == 2 and np.allclose(at.ϕ, [0.5, 0.5])
 
 
+def test_step_args_schemes(mpm):
+    # procedure!'s Val(:MUSL) / Val(:USL) / Val(:USF) map onto mpm_args.scheme
+    for name, code in (("MUSL", 0), ("USL", 1), ("USF", 2)):
+        args = mpm.scenes.column_collapse_2d(h=0.02, nsteps=2, scheme=name)[0]
+        assert mpm.HIPModel.step_args(args).scheme == code
+    args.coupling = "TS"
+    with pytest.raises(NotImplementedError, match="coupling"):
+        mpm.HIPModel.step_args(args)
+    args.coupling, args.scheme = "OS", "APIC"
+    with pytest.raises(NotImplementedError):
+        mpm.HIPModel.step_args(args)
+
+
 def test_non_hip_device_raises(mpm):
     args, grid, mp, attr, bc = mpm.scenes.column_collapse_2d(h=0.02, nsteps=2)
     with pytest.raises(NotImplementedError, match="HIP"):
@@ -158,3 +171,27 @@ def test_snapshot_layout_and_cadence(mpm, monkeypatch, tmp_path):
     names = [c[0] for c in rec.calls]
     # one download per group plus the final one
     assert names.count("mpm_download_particles") == want + 1
+
+
+def test_snapshot_state_for_resume(mpm):
+    # the arrays a resume uploads from one snapshot group: F carries det F =
+    # volume / Ω0 (the reference saves no F), ρs = mass / volume, ρs0 = mass / Ω0
+    rng = np.random.default_rng(3)
+    for dim in (2, 3):
+        n, ns = 50, 4 if dim == 2 else 6
+        g = dict(coords=rng.normal(size=(n, dim)), velocity_s=rng.normal(size=(n, dim)),
+                 stress=rng.normal(size=(n, ns)), strain_s=rng.normal(size=(n, ns)),
+                 eqstrain=rng.random(n), ekstrain=rng.random(n), eqrate=rng.random(n),
+                 volume=rng.uniform(0.5, 2.0, n), mass_s=rng.uniform(1.0, 3.0, n),
+                 time=np.asarray(0.125))
+        vol0 = rng.uniform(0.5, 2.0, n)
+        arrs, Ti = mpm.HIPModel.snapshot_state(g, vol0, np.ones(n))
+        assert Ti == 0.125
+        F = arrs["F"].reshape(n, dim, dim)
+        np.testing.assert_allclose(np.linalg.det(F), g["volume"] / vol0, rtol=1e-14)
+        np.testing.assert_allclose(arrs["rho"] * arrs["omega"], g["mass_s"], rtol=1e-15)
+        np.testing.assert_allclose(arrs["rho0"] * arrs["omega0"], g["mass_s"], rtol=1e-15)
+        for k, src in (("xi", "coords"), ("vs", "velocity_s"), ("sigma", "stress"),
+                       ("eps_s", "strain_s"), ("eps_q", "eqstrain"), ("eps_k", "ekstrain")):
+            np.testing.assert_array_equal(arrs[k], g[src])
+            assert arrs[k].flags.f_contiguous
