@@ -496,7 +496,7 @@ struct Impl : ImplBase {
         sa.pic = R(ar.pic);
         sa.zeta = R(ar.zeta_s);
         sa.va = ar.va;
-        sa.gmode = ar.va == MPM_VA_V ? 1 : (ar.scheme == MPM_MUSL ? 0 : 2);
+        sa.pic_new = (ar.va == MPM_VA_A && ar.scheme != MPM_MUSL) ? 1 : 0;
         sa.remap = ar.scheme == MPM_MUSL ? 1 : 0;
         // a USF context's first stress update has no position increment to commit
         sa.commit_dx = (phase != PH_HEAD || usf_pending) ? 1 : 0;

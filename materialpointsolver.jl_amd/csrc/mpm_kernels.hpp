@@ -130,7 +130,7 @@ template <typename R>
 struct StepArgs {
     R dt, G, flip, pic, zeta;
     int va;            // 0 = :a, 1 = :v
-    int gmode;         // re-map gather: 0 MUSL :a, 1 :v, 2 USL/USF :a (PIC on the new velocity)
+    int pic_new;       // USL/USF :a: the PIC term gathers vsT (the updated velocity), not vs
     int remap;         // 1: MUSL (momentum re-map); 0: USL/USF (grid solve #1 writes du)
     int commit_dx;     // k_g2p2 adds the step's position increment (0: first USF step)
     int plastic;       // Ti >= Te
@@ -456,18 +456,18 @@ __global__ __launch_bounds__(256) void k_grid1(GridGeom g, GridPtrs<R> gp, StepA
 // MUSL re-map (doublemapping1(_a)_OS! + doublemapping2_OS!, utils_OS.jl:698-959)
 // ---------------------------------------------------------------------------
 // One node's share of the re-map sums: A = Σ N vsT, and for :a B = Σ N as,
-// C = Σ N vs (MUSL, gmode 0) or Σ N vsT (USL/USF, gmode 2: the reference's
-// doublemapping1_a_USL_OS! gathers the already updated nodal velocity,
-// OS_USL.jl:271-313); for :v (gmode 1) B = Σ N (vsT - vs), C = Σ N vsT.
+// C = Σ N vs; for :v B = Σ N (vsT - vs), C = Σ N vsT.  (USL/USF :a gather
+// vsT in place of vs, see pic_field: doublemapping1_a_USL_OS! reads the
+// already updated nodal velocity, OS_USL.jl:271-313.)
 template <int DIM, typename R>
-__device__ __forceinline__ void g2p1_accumulate(int gmode, R N, const R* vT, const R* vA,
-                                                const R* vS, R* A, R* B, R* C) {
+__device__ __forceinline__ void g2p1_accumulate(int va, R N, const R* vT, const R* vA, const R* vS,
+                                                R* A, R* B, R* C) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        if (gmode != 1) {
+        if (va == 0) {
             A[d] += N * vT[d];
             B[d] += N * vA[d];
-            C[d] += N * (gmode == 2 ? vT[d] : vS[d]);
+            C[d] += N * vS[d];
         } else {
             A[d] += N * vT[d];
             B[d] += N * (vT[d] - vS[d]);
@@ -484,12 +484,18 @@ __device__ __forceinline__ void g2p1_commit(PartPtrs<R>& pp, const StepArgs<R>& 
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         R v = pp.v[d][i];
-        R vn = (sa.gmode != 1) ? sa.flip * v + sa.pic * C[d] + B[d] * sa.dt
-                               : sa.flip * (v + B[d]) + sa.pic * C[d];
+        R vn = (sa.va == 0) ? sa.flip * v + sa.pic * C[d] + B[d] * sa.dt
+                            : sa.flip * (v + B[d]) + sa.pic * C[d];
         pp.v[d][i] = vn;
         pp.dxp[d][i] = A[d] * sa.dt;
         pv[d] = m * vn;
     }
+}
+
+// The grid field whose gather feeds the re-map's C sum (see g2p1_accumulate).
+template <typename R>
+__device__ __forceinline__ R* const* pic_field(const GridPtrs<R>& gp, const StepArgs<R>& sa) {
+    return sa.pic_new ? gp.vsT : gp.vs;
 }
 
 template <int DIM, int S, typename R>
@@ -507,6 +513,7 @@ __global__ __launch_bounds__(256) void k_g2p1_p2g2(GridGeom g, PartPtrs<R> pp, G
         return;  // already reported by k_p2g
     }
     R A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, C[3] = {0, 0, 0};
+    R* const* vsf = pic_field(gp, sa);
     for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
         if (N != R(0)) {
             R vT[3], vA[3], vS[3];
@@ -514,9 +521,9 @@ __global__ __launch_bounds__(256) void k_g2p1_p2g2(GridGeom g, PartPtrs<R> pp, G
             for (int d = 0; d < DIM; ++d) {
                 vT[d] = gp.vsT[d][n];
                 vA[d] = gp.as[d][n];
-                vS[d] = gp.vs[d][n];
+                vS[d] = vsf[d][n];
             }
-            g2p1_accumulate<DIM, R>(sa.gmode, N, vT, vA, vS, A, B, C);
+            g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
         }
     });
     R pv[3];

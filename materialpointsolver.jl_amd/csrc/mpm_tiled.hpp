@@ -3,9 +3,9 @@
 // After every sort (mpm_sort.hpp) the particles of each tile of stencil
 // bases (4x4x4 in 3D, 8x8 in 2D) are contiguous, ordered by (rank in the
 // dual cell, base).  One wavefront owns one tile at a time and an LDS window
-// of the grid nodes the tile's stencils reach, with one base of slack on every
-// side for particles that drifted since the sort (window edge = tile edge + 2
-// + S - 1 nodes):
+// of the grid nodes the tile's stencils reached at the sort (window edge =
+// tile edge + S - 1 nodes; kWinSlack bases of slack per side would widen it
+// for particles that drift):
 //
 //  * scatters (P2G #1, P2G #2) accumulate into the window and flush it with
 //    one global atomic per touched node and field;
@@ -20,7 +20,8 @@
 // read-add-write (volatile, so the slots stay in program order).  LDS float
 // atomics would serialise per lane on this hardware.  A particle that lost its
 // claim or drifted out of the window scatters straight to the grid with
-// global atomics, so results never depend on the sort interval.
+// global atomics (and a gather reads such a particle's stencil from the grid),
+// so results never depend on the sort interval.
 //
 // Tile entries are dealt to the 8 XCDs in contiguous chunks (a workgroup's
 // XCD is its blockIdx modulo 8), so neighbouring tiles, which share halo
@@ -35,6 +36,12 @@ namespace mpm {
 constexpr int kRankBits = 5;   // ranks 0..31 (a fuller dual cell shares rank 31)
 constexpr int kLocalBits = 6;  // 64 bases per tile
 constexpr int kTileShift = kRankBits + kLocalBits;
+// Bases of slack on each side of a window.  0 measured fastest on the 3D
+// column: with one base of slack the 7-field scatter window grows from 6^3 to
+// 8^3 nodes, fewer wavefronts fit per CU, and P2G #1 and the re-map gather
+// ran 6-7 % slower, while the few particles that drift out of a slack-free
+// window between sorts cost little on the fallback path.
+constexpr int kWinSlack = 0;
 
 struct TileGeom {
     int tsh;         // log2 of the tile edge in stencil bases
@@ -43,12 +50,13 @@ struct TileGeom {
 };
 
 // Window geometry of a tile: NB bases and WW nodes per axis, x fastest, z
-// stride ZS.  (A z stride padded to 4 mod 32, which spreads a 4x4x4 block of
-// bases over all LDS banks, measured slower on the 3D column: the larger
-// scatter window costs a resident wavefront per CU.  ZS stays WW * WW.)
+// stride ZS.  (Row and plane strides padded so that a round's 4x4x4 block of
+// bases hits distinct LDS banks measured slower on the 3D column: the larger
+// window costs resident wavefronts, and bank conflicts are not what bounds the
+// scatter.  ZS stays WW * WW.)
 template <int DIM, int S, int TSH>
 struct Win {
-    static constexpr int NB = (1 << TSH) + 2;
+    static constexpr int NB = (1 << TSH) + 2 * kWinSlack;
     static constexpr int WW = NB + S - 1;
     static constexpr int ZS = WW * WW;
     static constexpr int WN = DIM == 3 ? ZS * WW : WW * WW;
@@ -64,15 +72,15 @@ __device__ __forceinline__ void xcd_chunk(int count, int& e0, int& e1, int& step
     step = nloc;
 }
 
-// First window node of a tile (one base of slack below the tile).
+// First window node of a tile (kWinSlack bases below the tile).
 template <int DIM>
 __device__ __forceinline__ void tile_window_origin(const TileGeom& tg, int tile, int* w0) {
     const int cx = tile % tg.tx;
     const int cy = (tile / tg.tx) % tg.ty;
     const int cz = DIM == 3 ? tile / (tg.tx * tg.ty) : 0;
-    w0[0] = (cx << tg.tsh) - 1;
-    w0[1] = (cy << tg.tsh) - 1;
-    w0[2] = DIM == 3 ? (cz << tg.tsh) - 1 : 0;
+    w0[0] = (cx << tg.tsh) - kWinSlack;
+    w0[1] = (cy << tg.tsh) - kWinSlack;
+    w0[2] = DIM == 3 ? (cz << tg.tsh) - kWinSlack : 0;
 }
 
 // LDS-typed pointers: volatile accesses through a generic pointer would be
@@ -159,6 +167,54 @@ __device__ __forceinline__ void p2g_values(R N, R dNx, R dNy, R dNz, R m, R vol,
         c[5] = -vol * (dNy * sg[1] + dNx * sg[3] + dNz * sg[4]) + NiM * G.g[1];
         c[6] = -vol * (dNz * sg[2] + dNx * sg[5] + dNy * sg[4]) + NiM * G.g[2];
     }
+}
+
+// P2G #1 into a window, 3D, with the force factored per (y, z) node pair:
+// with A = Ny Nz, B = dNy Nz, C = Ny dNz the gradient is (dNx A, Nx B, Nx C),
+// so f_k = dNx P_k + Nx Q_k with P = -Ω σ_{k x} A and
+// Q_k = -Ω (σ_{k y} B + σ_{k z} C) + m A g_k (the same terms as p2g_values,
+// utils_OS.jl:480-494, grouped to take the products out of the x loop).
+// Slots stay in program order with each slot's 7 reads before its 7 writes.
+template <int S, typename R, typename Win>
+__device__ __forceinline__ void p2g3_window(const Stencil<3, S, R>& st, R m, R vol, const R* pv,
+                                            const R* sg, const Gravity<R>& G, Win L0, int ws,
+                                            int zs, int wn) {
+    const R mv = -vol;
+    const R sxx = mv * sg[0], sxy = mv * sg[3], sxz = mv * sg[5];
+    const R syy = mv * sg[1], syz = mv * sg[4], szz = mv * sg[2];
+    static_for<S>([&](auto c) {
+        static_for<S>([&](auto b) {
+            const R A = st.N[1][b] * st.N[2][c];
+            const R B = st.dN[1][b] * st.N[2][c];
+            const R C = st.N[1][b] * st.dN[2][c];
+            const R mA = m * A;
+            const R Px = sxx * A, Py = sxy * A, Pz = sxz * A;
+            const R Qx = sxy * B + sxz * C + mA * G.g[0];
+            const R Qy = syy * B + syz * C + mA * G.g[1];
+            const R Qz = syz * B + szz * C + mA * G.g[2];
+            static_for<S>([&](auto a) {
+                const R Nx = st.N[0][a], dNx = st.dN[0][a];
+                const R N = Nx * A;
+                if (N != R(0)) {
+                    R cv[7];
+                    cv[0] = m * N;
+                    cv[1] = N * pv[0];
+                    cv[2] = N * pv[1];
+                    cv[3] = N * pv[2];
+                    cv[4] = dNx * Px + Nx * Qx;
+                    cv[5] = dNx * Py + Nx * Qy;
+                    cv[6] = dNx * Pz + Nx * Qz;
+                    auto L = L0 + (int(c) * zs + int(b) * ws + int(a));
+                    R old[7];
+#pragma unroll
+                    for (int f = 0; f < 7; ++f) old[f] = L[f * wn];
+#pragma unroll
+                    for (int f = 0; f < 7; ++f) L[f * wn] = old[f] + cv[f];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            });
+        });
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -276,7 +332,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     if (inside) claim[bidx] = tag;
                     const bool won = inside && claim[bidx] == tag;
                     tag += 64u;
-                    if (won) {
+                    if (won && MODE == 0 && DIM == 3) {
+                        if constexpr (MODE == 0 && DIM == 3)
+                            p2g3_window<S, R>(st, m, vol, pv, P.sg, G, win + wb, WW, ZS, WN);
+                    } else if (won) {
                         for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R dNx, R dNy, R dNz) {
                             if (N != R(0)) {
                                 R c[NF];
@@ -412,14 +471,15 @@ __global__ __launch_bounds__(256) void k_g2p1_tiled(GridGeom g, TileGeom tg, Par
                                                    const int* __restrict__ count_ptr) {
     using W = Win<DIM, S, TSH>;
     constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
-    constexpr int NG = 3 * DIM;  // gathered: vsT, as, vs
+    constexpr int NG = 3 * DIM;  // gathered: vsT, as, vs (or vsT again, see pic_field)
     __shared__ R gwin[NG * WN];
+    R* const* vsf = pic_field(gp, sa);
     R* gfields[NG];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         gfields[d] = gp.vsT[d];
         gfields[DIM + d] = gp.as[d];
-        gfields[2 * DIM + d] = gp.vs[d];
+        gfields[2 * DIM + d] = vsf[d];
     }
     int e0, e1, est;
     xcd_chunk(*count_ptr, e0, e1, est);
@@ -458,7 +518,7 @@ __global__ __launch_bounds__(256) void k_g2p1_tiled(GridGeom g, TileGeom tg, Par
                         vA[d] = L[(DIM + d) * WN];
                         vS[d] = L[(2 * DIM + d) * WN];
                     }
-                    g2p1_accumulate<DIM, R>(sa.gmode, N, vT, vA, vS, A, B, C);
+                    g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
                 });
             } else {
                 for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
@@ -468,9 +528,9 @@ __global__ __launch_bounds__(256) void k_g2p1_tiled(GridGeom g, TileGeom tg, Par
                         for (int d = 0; d < DIM; ++d) {
                             vT[d] = gp.vsT[d][n];
                             vA[d] = gp.as[d][n];
-                            vS[d] = gp.vs[d][n];
+                            vS[d] = vsf[d][n];
                         }
-                        g2p1_accumulate<DIM, R>(sa.gmode, N, vT, vA, vS, A, B, C);
+                        g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
                     }
                 });
             }
