@@ -23,6 +23,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -69,11 +70,30 @@ __global__ __launch_bounds__(256) void k_sort_keys(GridGeom g, SortKeyGeom kg,
     idx[i] = i;
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_gather(const T* __restrict__ src, const int* __restrict__ idx,
-                                                T* __restrict__ dst, int np) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < np) dst[i] = src[idx[i]];
+// Every permuted particle stream in one launch: dst_k[i] = src_k[order[i]]
+// (bit copies of 4- or 8-byte elements).  One index read per particle and
+// all streams' loads in flight together, instead of one launch per stream.
+struct PermuteStreams {
+    static constexpr int kMax = 48;
+    const void* src[kMax];
+    void* dst[kMax];
+    int wide[kMax];  // 1: 8-byte elements, 0: 4-byte
+    int count;
+};
+
+__global__ __launch_bounds__(256) void k_permute_all(PermuteStreams a, const int* __restrict__ order,
+                                                     int np) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const int j = order[i];
+#pragma unroll 4
+    for (int k = 0; k < a.count; ++k) {
+        if (a.wide[k])
+            static_cast<unsigned long long*>(a.dst[k])[i] =
+                static_cast<const unsigned long long*>(a.src[k])[j];
+        else
+            static_cast<unsigned*>(a.dst[k])[i] = static_cast<const unsigned*>(a.src[k])[j];
+    }
 }
 
 __global__ __launch_bounds__(256) void k_fill_u32(unsigned* __restrict__ p, long long n) {
@@ -151,6 +171,12 @@ struct SortState {
         }
         e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys, keys_alt, idx, idx_alt,
                                                int(np), 0, key_bits);
+        // the same workspace serves the compaction of the non-empty tile list
+        size_t sel_bytes = 0;
+        if (e == hipSuccess)
+            e = hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, hipcub::CountingInputIterator<int>(0),
+                                              nonempty, tile_list, tile_count, tg.ntiles);
+        if (sel_bytes > temp_bytes) temp_bytes = sel_bytes;
         if (e == hipSuccess) e = hipMalloc(&temp, temp_bytes ? temp_bytes : 1);
         if (e != hipSuccess) {
             error = std::string("radix sort workspace: ") + hipGetErrorString(e);
@@ -171,17 +197,6 @@ struct SortState {
         nonempty = nullptr;
     }
 
-    template <typename T>
-    hipError_t permute(T*& arr, void*& other, const int* order, hipStream_t s) {
-        const unsigned nb = unsigned((np + 255) / 256);
-        k_gather<T><<<nb, 256, 0, s>>>(arr, order, (T*)other, int(np));
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        T* old = arr;
-        arr = (T*)other;
-        other = (void*)old;
-        return hipSuccess;
-    }
 
     // Every stream the sort permutes, in a fixed order: f(pointer, element size).
     template <typename F>
@@ -224,23 +239,33 @@ struct SortState {
         });
         if (rc != MPM_OK) return rc;
         const int* order = idx_alt;
-        hipError_t e = hipSuccess;
-        rc = c->launch(7, [&] {
-            size_t k = 0;
-            for_each_stream(pp, dim, [&](auto*& ptr) {
-                if (e == hipSuccess) e = permute(ptr, alt[k], order, s);
-                ++k;
-            });
+        PermuteStreams ps{};
+        for_each_stream(pp, dim, [&](auto*& ptr) {
+            ps.src[ps.count] = ptr;
+            ps.dst[ps.count] = alt[ps.count];
+            ps.wide[ps.count] = sizeof(*ptr) == 8 ? 1 : 0;
+            ++ps.count;
         });
+        rc = c->launch(7, [&] { k_permute_all<<<nb, 256, 0, s>>>(ps, order, n); });
         if (rc != MPM_OK) return rc;
-        if (e != hipSuccess) return c->hip_fail(e, "k_permute");
+        // the permuted copies become the particle streams; the old ones the spares
+        size_t k = 0;
+        for_each_stream(pp, dim, [&](auto*& ptr) {
+            using T = std::remove_reference_t<decltype(*ptr)>;
+            void* old = (void*)ptr;
+            ptr = static_cast<T*>(alt[k]);
+            alt[k] = old;
+            ++k;
+        });
         parity ^= 1;
         // tile segments of the new order and the ordered list of non-empty tiles
         rc = c->launch(6, [&] {
             const unsigned tb = unsigned((tg.ntiles + 255) / 256);
             k_fill_u8<<<tb, 256, 0, s>>>(nonempty, tg.ntiles);
             k_tile_segments<<<nb, 256, 0, s>>>(keys_alt, n, kTileShift, seg_begin, seg_end, nonempty);
-            k_compact_tiles<<<1, 1024, 0, s>>>(nonempty, tg.ntiles, tile_list, tile_count);
+            size_t bytes = temp_bytes;
+            hipcub::DeviceSelect::Flagged(temp, bytes, hipcub::CountingInputIterator<int>(0),
+                                          nonempty, tile_list, tile_count, tg.ntiles, s);
         });
         if (rc != MPM_OK) return rc;
         tiles_valid = true;

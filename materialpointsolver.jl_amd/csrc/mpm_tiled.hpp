@@ -564,29 +564,4 @@ __global__ __launch_bounds__(256) void k_fill_u8(unsigned char* __restrict__ p, 
     if (i < n) p[i] = 0;
 }
 
-// Ordered compaction of the non-empty tiles (one workgroup of 1024 threads).
-__global__ __launch_bounds__(1024) void k_compact_tiles(const unsigned char* __restrict__ nonempty,
-                                                        int ntiles, int* __restrict__ list,
-                                                        int* __restrict__ count) {
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
-    const int per = (ntiles + 1023) / 1024;
-    const int b = t * per, e = min(ntiles, b + per);
-    int c = 0;
-    for (int k = b; k < e; ++k) c += nonempty[k];
-    part[t] = c;
-    __syncthreads();
-    // inclusive Hillis-Steele scan
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    int pos = part[t] - c;
-    for (int k = b; k < e; ++k)
-        if (nonempty[k]) list[pos++] = k;
-    if (t == 1023) *count = part[1023];
-}
-
 }  // namespace mpm
