@@ -224,6 +224,7 @@ struct SortState {
         if (!enabled) return MPM_OK;
         const unsigned nb = unsigned((np + 255) / 256);
         const int n = int(np);
+        hipError_t lib = hipSuccess;  // hipcub's own status (its launches are checked by launch())
         int rc = c->launch(6, [&] {
             k_fill_u32<<<unsigned((nbases + 255) / 256), 256, 0, s>>>(count, nbases);
             const double* x2 = dim == 3 ? pp.x[2] : pp.x[1];
@@ -234,10 +235,11 @@ struct SortState {
                 if (S == 3) k_sort_keys<2, 3><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
                 else k_sort_keys<2, 2><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
             }
-            hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_alt, idx, idx_alt, n, 0,
-                                               key_bits, s);
+            lib = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_alt, idx, idx_alt, n,
+                                                     0, key_bits, s);
         });
         if (rc != MPM_OK) return rc;
+        if (lib != hipSuccess) return c->hip_fail(lib, "radix sort");
         const int* order = idx_alt;
         PermuteStreams ps{};
         for_each_stream(pp, dim, [&](auto*& ptr) {
@@ -264,10 +266,11 @@ struct SortState {
             k_fill_u8<<<tb, 256, 0, s>>>(nonempty, tg.ntiles);
             k_tile_segments<<<nb, 256, 0, s>>>(keys_alt, n, kTileShift, seg_begin, seg_end, nonempty);
             size_t bytes = temp_bytes;
-            hipcub::DeviceSelect::Flagged(temp, bytes, hipcub::CountingInputIterator<int>(0),
-                                          nonempty, tile_list, tile_count, tg.ntiles, s);
+            lib = hipcub::DeviceSelect::Flagged(temp, bytes, hipcub::CountingInputIterator<int>(0),
+                                                nonempty, tile_list, tile_count, tg.ntiles, s);
         });
         if (rc != MPM_OK) return rc;
+        if (lib != hipSuccess) return c->hip_fail(lib, "tile list compaction");
         tiles_valid = true;
         return MPM_OK;
     }

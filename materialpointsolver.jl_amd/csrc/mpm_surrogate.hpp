@@ -130,9 +130,9 @@ struct mpm_mlp {
         return e;
     }
     void release() {
-        for (void* p : owned) hipFree(p);
+        for (void* p : owned) (void)hipFree(p);
         owned.clear();
-        if (stream) hipStreamDestroy(stream);
+        if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
 
@@ -176,8 +176,8 @@ struct mpm_mlp {
     int upload_data(const float* x, const float* y, long long n) {
         using namespace mpm::mlp;
         if (n > data_cap) {
-            if (X) hipFree(X);
-            if (Y) hipFree(Y);
+            if (X) (void)hipFree(X);
+            if (Y) (void)hipFree(Y);
             X = Y = nullptr;
             hipError_t e = hipMalloc((void**)&X, sizeof(float) * n * NIN);
             if (e == hipSuccess) e = hipMalloc((void**)&Y, sizeof(float) * n * NOUT);
@@ -275,9 +275,9 @@ int mpm_mlp_create(int64_t batch_capacity, mpm_mlp** out) {
 
 void mpm_mlp_destroy(mpm_mlp* m) {
     if (!m) return;
-    if (m->stream) hipStreamSynchronize(m->stream);
-    if (m->X) hipFree(m->X);
-    if (m->Y) hipFree(m->Y);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    if (m->X) (void)hipFree(m->X);
+    if (m->Y) (void)hipFree(m->Y);
     m->release();
     delete m;
 }
@@ -322,7 +322,7 @@ int mpm_mlp_forward(mpm_mlp* m, const float* x, int64_t n, float* y) {
     if (e == hipSuccess)
         e = hipMemcpyAsync(y, dy, sizeof(float) * n * NOUT, hipMemcpyDeviceToHost, m->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
-    hipFree(dy);
+    (void)hipFree(dy);
     return e == hipSuccess ? MPM_OK : m->hip_fail(e, "mpm_mlp_forward");
 }
 
