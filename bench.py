@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs (FETCH_SIZE, WRITE_SIZE)")
     ap.add_argument("--no-autotune", action="store_true", help="skip the autotuner report")
+    ap.add_argument("--no-full-run", action="store_true",
+                    help="skip the whole 20,000-step collapse (captured graphs, no events)")
+    ap.add_argument("--full-steps", type=int, default=20_000)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--ens-instances", type=int, default=256)
     ap.add_argument("--ens-steps", type=int, default=200)
@@ -150,6 +153,44 @@ def measure_traffic(a):
             out["kernels"][k] = {"fetch_bytes": fe[k] * 1024.0 * f_fetch,
                                  "write_bytes": wr[k] * 1024.0 * f_write}
     return out, ""
+
+
+def full_run_leg(a, mpm):
+    """The whole collapse from rest, 20,000 steps (the horizon the 3D column is
+    defined with), with captured step graphs and no per-kernel events: the
+    wall-clock cost of the job.  The spreading deposit costs more per step than
+    the first steps that the timed region above covers (fewer particles per
+    tile, rounds with fewer lanes), so this is the job-level figure."""
+    import numpy as np
+    args, grid, mp, attr, bc = mpm.scenes.column_collapse_3d(h=a.h, eps=a.precision)
+    dt = float(args.ΔT)
+    m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=a.sort_interval)
+    try:
+        sa = m.step_args(args)
+        seg, Ti, done = [], 0.0, 0
+        m.sync()
+        t0 = time.perf_counter()
+        while done < a.full_steps:
+            k = min(2000, a.full_steps - done)
+            ts = time.perf_counter()
+            Ti = m.run(sa, dt, Ti, k, graph=True)
+            m.sync()
+            seg.append(round((time.perf_counter() - ts) / k * 1e3, 4))
+            done += k
+            print(f"# full run: {done} steps", file=sys.stderr, flush=True)
+        T = time.perf_counter() - t0
+        x = m.download()["xi"]
+    finally:
+        m.close()
+    n = int(mp.np)
+    x0 = np.asarray(mp.ξ, dtype=np.float64)
+    res = {"steps": a.full_steps, "seconds": T, "ms_per_step": T / a.full_steps * 1e3,
+           "particle_steps_per_s": n * a.full_steps / T, "sort_interval": a.sort_interval,
+           "graph": True, "ms_per_step_per_2000": seg,
+           "runout_m": float(np.abs(x[:, :2]).max() - np.abs(x0[:, :2]).max())}
+    print(f"# full run: {a.full_steps} steps in {T:.2f} s, {res['ms_per_step']:.4f} ms/step, "
+          f"{res['particle_steps_per_s']:.3e} particle-steps/s", file=sys.stderr, flush=True)
+    return res
 
 
 def main():
@@ -274,6 +315,8 @@ def main():
         "cpu_baseline": cpu,
         "autotune": tuned,
     }
+    if not a.no_full_run:
+        out["full_run"] = full_run_leg(a, mpm)
     if not a.no_ensemble:
         out["batched"] = ensemble_leg(a, mpm)
     if not a.no_surrogate:

@@ -13,7 +13,9 @@
 //    window in LDS and then read every stencil from there.
 //
 // Scatter rounds.  A round is at most 64 particles of one rank of one tile,
-// so at the last sort their bases were pairwise distinct.  Each lane claims
+// so at the last sort their bases were pairwise distinct; when a rank group
+// leaves lanes free, the round continues with the next group, which adds into
+// a second window (kScatterWindows), so half-empty groups share a wavefront.  Each lane claims
 // its current base in a small LDS table (write a unique tag, read it back) to
 // confirm that this still holds; the winners' targets are then distinct at
 // every stencil slot and they add into the window with plain LDS
@@ -42,6 +44,12 @@ constexpr int kTileShift = kRankBits + kLocalBits;
 // ran 6-7 % slower, while the few particles that drift out of a slack-free
 // window between sorts cost little on the fallback path.
 constexpr int kWinSlack = 0;
+// Scatter windows per wavefront: a round is one rank group in window 0 plus,
+// in the lanes it leaves free, the next group in window 1 (the two are summed
+// before the flush).  Measured on the 3D column: P2G #1 161 -> 150 us early in
+// the collapse (52.8 particles per group) and 386 -> 343 us at step 5000 (about
+// 30 per group, as the deposit spreads and compacts unevenly).
+constexpr int kScatterWindows = 2;
 
 struct TileGeom {
     int tsh;         // log2 of the tile edge in stencil bases
@@ -232,8 +240,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     using W = Win<DIM, S, TSH>;
     constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
     constexpr int NF = MODE == 0 ? 1 + 2 * DIM : DIM;
-    __shared__ R win_[NF * WN];
-    __shared__ unsigned claim_[W::NBN];
+    constexpr int NWIN = kScatterWindows;
+    static_assert(NWIN == 1 || NWIN == 2, "one or two scatter windows");
+    __shared__ R win_[NWIN * NF * WN];
+    __shared__ unsigned claim_[NWIN * W::NBN];
     lds_ptr<R> win = (lds_ptr<R>)win_;
     lds_ptr<unsigned> claim = (lds_ptr<unsigned>)claim_;
     R* fields[NF];
@@ -257,16 +267,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const int beg = seg_begin[tile], end = seg_end[tile];
         int w0[3];
         tile_window_origin<DIM>(tg, tile, w0);
-        for (int k = threadIdx.x; k < NF * WN; k += 64) win[k] = R(0);
+        for (int k = threadIdx.x; k < NWIN * NF * WN; k += 64) win[k] = R(0);
         // Rounds are software-pipelined: the keys of the next round come from a
         // two-register sliding window over the sorted keys (a round has at most
         // 64 particles, so the next round lies in [cur, cur + 128)), and the next
         // round's particle data is loaded before this round's LDS scatter.
         const int lane = threadIdx.x;
-        auto member = [&](unsigned ka, int c0, bool& act) -> int {
+        // A round: the keys up to the end of the first rank group (window 0)
+        // and, with two windows, on to the end of the second group (window 1).
+        // Returns the round's particle count; side is the lane's window.
+        auto member = [&](unsigned ka, int c0, bool& act, int& side) -> int {
             const unsigned grp = unsigned(__shfl(int(ka), 0)) >> kLocalBits;
-            act = (c0 + lane < end) && (ka >> kLocalBits) == grp;
-            return __popcll(__ballot(act));
+            const bool in = c0 + lane < end;
+            const bool a0 = in && (ka >> kLocalBits) == grp;
+            const int n0 = __popcll(__ballot(a0));
+            side = 0;
+            act = a0;
+            if (NWIN == 1 || n0 >= 64) return n0;
+            const unsigned grp2 = unsigned(__shfl(int(ka), n0)) >> kLocalBits;
+            const bool a1 = in && lane >= n0 && (ka >> kLocalBits) == grp2;
+            if (a1) side = 1;
+            act = a0 || a1;
+            return n0 + __popcll(__ballot(a1));
         };
         struct Pre {
             double x[3];
@@ -297,7 +319,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         unsigned kA = cur + lane < end ? keys[cur + lane] : 0u;
         unsigned kB = cur + 64 + lane < end ? keys[cur + 64 + lane] : 0u;
         bool act = false;
-        int cnt = cur < end ? member(kA, cur, act) : 0;
+        int side = 0;
+        int cnt = cur < end ? member(kA, cur, act, side) : 0;
         Pre P{};
         load_pre(cur + lane, act, P);
         while (cur < end) {
@@ -307,7 +330,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const int curn = cur + cnt;
             const unsigned kBn = curn + 64 + lane < end ? keys[curn + 64 + lane] : 0u;
             bool actn = false;
-            const int cntn = curn < end ? member(kAn, curn, actn) : 0;
+            int siden = 0;
+            const int cntn = curn < end ? member(kAn, curn, actn, siden) : 0;
             Pre Pn{};
             load_pre(curn + lane, actn, Pn);
             if (act) {
@@ -329,6 +353,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     };
                     int wb, bidx;
                     const bool inside = window_slot<DIM, NB, WW, ZS>(st.base, w0, wb, bidx);
+                    bidx += side * W::NBN;
+                    wb += side * NF * WN;
                     if (inside) claim[bidx] = tag;
                     const bool won = inside && claim[bidx] == tag;
                     tag += 64u;
@@ -368,10 +394,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             }
             P = Pn;
             act = actn;
+            side = siden;
             cnt = cntn;
             cur = curn;
             kA = kAn;
             kB = kBn;
+        }
+        if constexpr (NWIN == 2) {
+            for (int k = threadIdx.x; k < NF * WN; k += 64) win[k] = win[k] + win[NF * WN + k];
         }
         flush_window<DIM, WW, ZS, WN, NF, R>(g, w0, win, fields);
     }
