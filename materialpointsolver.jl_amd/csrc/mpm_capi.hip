@@ -540,9 +540,14 @@ struct Impl : ImplBase {
             // new momentum (tiled: its own kernel; otherwise fused, global atomics)
             if ((rc = c->launch(KID_G2P1, [&] {
                      if (tiled)
-                         k_g2p1_tiled<DIM, S, TSH, R><<<sort.tiled_grid(), 256, 0, s>>>(
-                             geom, sort.tg, pp, gp, sa, sort.tile_list, sort.seg_begin,
-                             sort.seg_end, sort.tile_count);
+                         if (sa.va != 0 || sa.pic != R(0))
+                             k_g2p1_tiled<DIM, S, TSH, R, true><<<sort.tiled_grid(), 256, 0, s>>>(
+                                 geom, sort.tg, pp, gp, sa, sort.tile_list, sort.seg_begin,
+                                 sort.seg_end, sort.tile_count);
+                         else
+                             k_g2p1_tiled<DIM, S, TSH, R, false><<<sort.tiled_grid(), 256, 0, s>>>(
+                                 geom, sort.tg, pp, gp, sa, sort.tile_list, sort.seg_begin,
+                                 sort.seg_end, sort.tile_count);
                      else
                          k_g2p1_p2g2<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, sa, n, fl);
                  })) != MPM_OK)

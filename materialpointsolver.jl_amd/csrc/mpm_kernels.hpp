@@ -514,6 +514,7 @@ __global__ __launch_bounds__(256) void k_g2p1_p2g2(GridGeom g, PartPtrs<R> pp, G
     }
     R A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, C[3] = {0, 0, 0};
     R* const* vsf = pic_field(gp, sa);
+    const bool need_vs = sa.va != 0 || sa.pic != R(0);  // :a with PIC = 0 multiplies it by 0
     for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
         if (N != R(0)) {
             R vT[3], vA[3], vS[3];
@@ -521,7 +522,7 @@ __global__ __launch_bounds__(256) void k_g2p1_p2g2(GridGeom g, PartPtrs<R> pp, G
             for (int d = 0; d < DIM; ++d) {
                 vT[d] = gp.vsT[d][n];
                 vA[d] = gp.as[d][n];
-                vS[d] = vsf[d][n];
+                vS[d] = need_vs ? vsf[d][n] : R(0);
             }
             g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
         }

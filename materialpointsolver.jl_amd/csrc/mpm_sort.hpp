@@ -11,9 +11,12 @@
 // with tiles of 4x4x4 bases in 3D (8x8 in 2D) and the rank counted with one
 // integer atomic per particle.  Inside a tile, the particles of one rank then
 // have pairwise distinct bases, which is what the tiled scatter kernels need
-// (mpm_tiled.hpp).  The key sort is the rocPRIM radix sort (through hipCUB)
-// over (key, index) pairs; the permutation is one gather kernel per particle
-// stream into that stream's own alternate buffer, followed by a pointer swap,
+// (mpm_tiled.hpp).  The keys are placed by a counting sort: the per-base
+// counts give each tile's layout in (rank group, base) order (k_tile_layout),
+// a scan of the tile sizes gives the tiles' offsets, and every particle
+// computes its position from its tile, rank and base (k_sort_dest); no
+// comparison or radix passes.  The permutation is one gather kernel over all
+// particle streams into each stream's own alternate buffer, then a pointer swap,
 // so every stream flips between the same two buffers and the particle
 // pointers take one of two configurations (parity), which is what lets
 // captured step graphs be reused across sorts.  The original particle index
@@ -46,7 +49,7 @@ __global__ __launch_bounds__(256) void k_sort_keys(GridGeom g, SortKeyGeom kg,
                                                    const double* __restrict__ x2,
                                                    unsigned* __restrict__ count,
                                                    unsigned* __restrict__ keys,
-                                                   int* __restrict__ idx, int np) {
+                                                   int* __restrict__ rank_out, int np) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const double* xs[3] = {x0, x1, x2};
@@ -59,15 +62,101 @@ __global__ __launch_bounds__(256) void k_sort_keys(GridGeom g, SortKeyGeom kg,
         b[a] = int(f);
     }
     const unsigned lin = unsigned(b[0]) + unsigned(kg.nb[0]) * (unsigned(b[1]) + unsigned(kg.nb[1]) * unsigned(b[2]));
-    unsigned rank = atomicAdd(&count[lin], 1u);
-    if (rank > (1u << kRankBits) - 1) rank = (1u << kRankBits) - 1;
+    const unsigned rank = atomicAdd(&count[lin], 1u);
     const int tm = (1 << kg.tsh) - 1;
     const unsigned tile = unsigned(b[0] >> kg.tsh) +
                           unsigned(kg.tx) * (unsigned(b[1] >> kg.tsh) + unsigned(kg.ty) * unsigned(b[2] >> kg.tsh));
     const unsigned local = unsigned(b[0] & tm) +
                            unsigned(tm + 1) * (unsigned(b[1] & tm) + unsigned(tm + 1) * unsigned(b[2] & tm));
-    keys[i] = (tile << kTileShift) | (rank << kLocalBits) | local;
-    idx[i] = i;
+    keys[i] = (tile << kTileShift) | (min(rank, (1u << kRankBits) - 1) << kLocalBits) | local;
+    rank_out[i] = int(rank);
+}
+
+// Counting sort by key.  The keys' order inside a tile is (rank group, base):
+// for a rank r below the last group the r-th particle of base l goes to
+//   start + off[r] + popcount(mask[r] & bases below l),
+// with mask[r] the bases holding more than r particles and off[r] the
+// particles of the lower groups; the last group (ranks >= kLastRank) holds the
+// rest in base order, each base's block at off[kLastRank] + ovf[l].  One
+// wavefront per tile, a lane per base, from the per-base counts.
+constexpr int kLastRank = (1 << kRankBits) - 1;
+
+template <int DIM>
+__global__ __launch_bounds__(64) void k_tile_layout(SortKeyGeom kg, const unsigned* __restrict__ count,
+                                                    unsigned long long* __restrict__ masks,
+                                                    int* __restrict__ offs, int* __restrict__ ovf,
+                                                    int* __restrict__ totals, int ntiles) {
+    const int t = blockIdx.x, lane = threadIdx.x;
+    if (t >= ntiles) return;
+    const int tm = (1 << kg.tsh) - 1;
+    const int cx = t % kg.tx, cy = (t / kg.tx) % kg.ty, cz = DIM == 3 ? t / (kg.tx * kg.ty) : 0;
+    const int lx = lane & tm, ly = (lane >> kg.tsh) & tm, lz = DIM == 3 ? lane >> (2 * kg.tsh) : 0;
+    const int b0 = (cx << kg.tsh) + lx, b1 = (cy << kg.tsh) + ly, b2 = DIM == 3 ? (cz << kg.tsh) + lz : 0;
+    const bool valid = b0 < kg.nb[0] && b1 < kg.nb[1] && b2 < kg.nb[2];
+    const int c = valid ? int(count[b0 + kg.nb[0] * (b1 + kg.nb[1] * b2)]) : 0;
+    int acc = 0;
+    for (int r = 0; r < kLastRank; ++r) {
+        const unsigned long long m = __ballot(c > r);
+        if (m == 0ull) break;
+        if (lane == 0) {
+            masks[(long long)t * kLastRank + r] = m;
+            offs[(long long)t * (kLastRank + 1) + r] = acc;
+        }
+        acc += __popcll(m);
+    }
+    // the last group: exclusive prefix over bases of max(0, c - kLastRank)
+    const int o = c > kLastRank ? c - kLastRank : 0;
+    int incl = o;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d);
+        if (lane >= d) incl += v;
+    }
+    ovf[(long long)t * 64 + lane] = incl - o;
+    const int tail = __shfl(incl, 63);
+    if (lane == 0) {
+        offs[(long long)t * (kLastRank + 1) + kLastRank] = acc;
+        totals[t] = acc + tail;
+    }
+}
+
+// Tile segments of the sorted order from the scanned tile sizes.
+__global__ __launch_bounds__(256) void k_tile_segments(const int* __restrict__ starts,
+                                                       const int* __restrict__ totals, int ntiles,
+                                                       int* __restrict__ seg_begin,
+                                                       int* __restrict__ seg_end,
+                                                       unsigned char* __restrict__ nonempty) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    seg_begin[t] = starts[t];
+    seg_end[t] = starts[t] + totals[t];
+    nonempty[t] = totals[t] > 0 ? 1 : 0;
+}
+
+// Destination of every particle in the sorted order: order[pos] = i and the
+// sorted key at pos.
+__global__ __launch_bounds__(256) void k_sort_dest(const unsigned* __restrict__ keys,
+                                                   const int* __restrict__ rank,
+                                                   const int* __restrict__ starts,
+                                                   const unsigned long long* __restrict__ masks,
+                                                   const int* __restrict__ offs,
+                                                   const int* __restrict__ ovf,
+                                                   int* __restrict__ order,
+                                                   unsigned* __restrict__ keys_sorted, int np) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const unsigned key = keys[i];
+    const long long t = key >> kTileShift;
+    const int l = int(key & ((1u << kLocalBits) - 1));
+    const int r = rank[i];
+    int pos = starts[t];
+    if (r < kLastRank)
+        pos += offs[t * (kLastRank + 1) + r] +
+               __popcll(masks[t * kLastRank + r] & ((1ull << l) - 1ull));
+    else
+        pos += offs[t * (kLastRank + 1) + kLastRank] + ovf[t * 64 + l] + (r - kLastRank);
+    order[pos] = i;
+    keys_sorted[pos] = key;
 }
 
 // Every permuted particle stream in one launch: dst_k[i] = src_k[order[i]]
@@ -111,13 +200,15 @@ struct SortState {
     SortKeyGeom kg{};
     GridGeom geom{};
     TileGeom tg{};
-    int key_bits = 32;
     // tile segments of the sorted order (valid after the first sort)
     int *seg_begin = nullptr, *seg_end = nullptr, *tile_list = nullptr, *tile_count = nullptr;
     unsigned char* nonempty = nullptr;
     bool tiles_valid = false;
     unsigned *keys = nullptr, *keys_alt = nullptr, *count = nullptr;
-    int *idx = nullptr, *idx_alt = nullptr;
+    int *idx = nullptr, *idx_alt = nullptr;  // per-particle rank; the sorted order
+    // counting-sort layout per tile (k_tile_layout)
+    unsigned long long* masks = nullptr;
+    int *offs = nullptr, *ovf = nullptr, *totals = nullptr, *starts = nullptr;
     void* temp = nullptr;
     size_t temp_bytes = 0;
     // Alternate buffers, one per permuted stream in for_each_stream order;
@@ -148,8 +239,6 @@ struct SortState {
             error = "grid too large for 32-bit sort keys";
             return MPM_E_ARG;
         }
-        key_bits = 1;
-        while ((1ull << key_bits) < maxkey) ++key_bits;
         tg.tsh = kg.tsh;
         tg.tx = kg.tx;
         tg.ty = kg.ty;
@@ -165,21 +254,26 @@ struct SortState {
         if (e == hipSuccess) e = hipMalloc((void**)&keys_alt, sizeof(unsigned) * np);
         if (e == hipSuccess) e = hipMalloc((void**)&idx, sizeof(int) * np);
         if (e == hipSuccess) e = hipMalloc((void**)&idx_alt, sizeof(int) * np);
+        const long long nt = tg.ntiles;
+        if (e == hipSuccess) e = hipMalloc((void**)&masks, sizeof(unsigned long long) * nt * kLastRank);
+        if (e == hipSuccess) e = hipMalloc((void**)&offs, sizeof(int) * nt * (kLastRank + 1));
+        if (e == hipSuccess) e = hipMalloc((void**)&ovf, sizeof(int) * nt * 64);
+        if (e == hipSuccess) e = hipMalloc((void**)&totals, sizeof(int) * nt);
+        if (e == hipSuccess) e = hipMalloc((void**)&starts, sizeof(int) * nt);
         if (e != hipSuccess) {
             error = std::string("hipMalloc: ") + hipGetErrorString(e);
             return MPM_E_NOMEM;
         }
-        e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys, keys_alt, idx, idx_alt,
-                                               int(np), 0, key_bits);
-        // the same workspace serves the compaction of the non-empty tile list
-        size_t sel_bytes = 0;
+        // one workspace for the scan of the tile sizes and the tile-list compaction
+        size_t scan_bytes = 0, sel_bytes = 0;
+        e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, totals, starts, tg.ntiles);
         if (e == hipSuccess)
             e = hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, hipcub::CountingInputIterator<int>(0),
                                               nonempty, tile_list, tile_count, tg.ntiles);
-        if (sel_bytes > temp_bytes) temp_bytes = sel_bytes;
+        temp_bytes = scan_bytes > sel_bytes ? scan_bytes : sel_bytes;
         if (e == hipSuccess) e = hipMalloc(&temp, temp_bytes ? temp_bytes : 1);
         if (e != hipSuccess) {
-            error = std::string("radix sort workspace: ") + hipGetErrorString(e);
+            error = std::string("sort workspace: ") + hipGetErrorString(e);
             return MPM_E_NOMEM;
         }
         return MPM_OK;
@@ -187,11 +281,13 @@ struct SortState {
 
     void release() {
         void* ps[] = {keys, keys_alt, idx, idx_alt, temp, seg_begin, seg_end, tile_list,
-                      tile_count, nonempty, count};
+                      tile_count, nonempty, count, masks, offs, ovf, totals, starts};
         for (void* p : ps)
-            if (p) hipFree(p);
+            if (p) (void)hipFree(p);
         keys = keys_alt = count = nullptr;
         idx = idx_alt = nullptr;
+        masks = nullptr;
+        offs = ovf = totals = starts = nullptr;
         temp = nullptr;
         seg_begin = seg_end = tile_list = tile_count = nullptr;
         nonempty = nullptr;
@@ -231,15 +327,18 @@ struct SortState {
             if (dim == 3) {
                 if (S == 3) k_sort_keys<3, 3><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
                 else k_sort_keys<3, 2><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
+                k_tile_layout<3><<<unsigned(tg.ntiles), 64, 0, s>>>(kg, count, masks, offs, ovf, totals, tg.ntiles);
             } else {
                 if (S == 3) k_sort_keys<2, 3><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
                 else k_sort_keys<2, 2><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
+                k_tile_layout<2><<<unsigned(tg.ntiles), 64, 0, s>>>(kg, count, masks, offs, ovf, totals, tg.ntiles);
             }
-            lib = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_alt, idx, idx_alt, n,
-                                                     0, key_bits, s);
+            size_t bytes = temp_bytes;
+            lib = hipcub::DeviceScan::ExclusiveSum(temp, bytes, totals, starts, tg.ntiles, s);
+            k_sort_dest<<<nb, 256, 0, s>>>(keys, idx, starts, masks, offs, ovf, idx_alt, keys_alt, n);
         });
         if (rc != MPM_OK) return rc;
-        if (lib != hipSuccess) return c->hip_fail(lib, "radix sort");
+        if (lib != hipSuccess) return c->hip_fail(lib, "tile scan");
         const int* order = idx_alt;
         PermuteStreams ps{};
         for_each_stream(pp, dim, [&](auto*& ptr) {
@@ -263,8 +362,7 @@ struct SortState {
         // tile segments of the new order and the ordered list of non-empty tiles
         rc = c->launch(6, [&] {
             const unsigned tb = unsigned((tg.ntiles + 255) / 256);
-            k_fill_u8<<<tb, 256, 0, s>>>(nonempty, tg.ntiles);
-            k_tile_segments<<<nb, 256, 0, s>>>(keys_alt, n, kTileShift, seg_begin, seg_end, nonempty);
+            k_tile_segments<<<tb, 256, 0, s>>>(starts, totals, tg.ntiles, seg_begin, seg_end, nonempty);
             size_t bytes = temp_bytes;
             lib = hipcub::DeviceSelect::Flagged(temp, bytes, hipcub::CountingInputIterator<int>(0),
                                                 nonempty, tile_list, tile_count, tg.ntiles, s);

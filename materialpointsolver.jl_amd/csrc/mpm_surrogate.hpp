@@ -366,7 +366,7 @@ int mpm_mlp_train(mpm_mlp* m, const float* x, const float* y, int64_t n, int64_t
     if (e == hipSuccess && loss_history && steps)
         e = hipMemcpyAsync(loss_history, hist, sizeof(float) * steps, hipMemcpyDeviceToHost, m->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
-    hipFree(hist);
+    (void)hipFree(hist);
     return e == hipSuccess ? MPM_OK : m->hip_fail(e, "mpm_mlp_train");
 }
 
@@ -397,9 +397,9 @@ int mpm_mlp_bench(mpm_mlp* m, int64_t n, int reps, double* ms_per_launch) {
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float ms = 0.0f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    if (dy) hipFree(dy);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (dy) (void)hipFree(dy);
     if (e != hipSuccess) return m->hip_fail(e, "mpm_mlp_bench");
     *ms_per_launch = double(ms) / reps;
     return MPM_OK;

@@ -492,7 +492,9 @@ __global__ __launch_bounds__(256) void k_g2p2_tiled(
 // tile's particles in order (no rounds: gathers do not conflict); the new
 // momentum m v goes to pp.pm for the momentum scatter k_scatter_tiled<..., 1>.
 // ---------------------------------------------------------------------------
-template <int DIM, int S, int TSH, typename R>
+// VS: whether vs is gathered.  The :a form with PIC = 0 multiplies its sum by
+// zero, so that window drops a third of its fields (the default FLIP = 1).
+template <int DIM, int S, int TSH, typename R, bool VS>
 __global__ __launch_bounds__(256) void k_g2p1_tiled(GridGeom g, TileGeom tg, PartPtrs<R> pp,
                                                    GridPtrs<R> gp, StepArgs<R> sa,
                                                    const int* __restrict__ tile_list,
@@ -501,10 +503,10 @@ __global__ __launch_bounds__(256) void k_g2p1_tiled(GridGeom g, TileGeom tg, Par
                                                    const int* __restrict__ count_ptr) {
     using W = Win<DIM, S, TSH>;
     constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
-    constexpr int NG = 3 * DIM;  // gathered: vsT, as, vs (or vsT again, see pic_field)
+    constexpr int NG = (VS ? 3 : 2) * DIM;  // gathered: vsT, as[, vs (or vsT again, see pic_field)]
     __shared__ R gwin[NG * WN];
     R* const* vsf = pic_field(gp, sa);
-    R* gfields[NG];
+    R* gfields[3 * DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         gfields[d] = gp.vsT[d];
@@ -546,7 +548,7 @@ __global__ __launch_bounds__(256) void k_g2p1_tiled(GridGeom g, TileGeom tg, Par
                     for (int d = 0; d < DIM; ++d) {
                         vT[d] = L[d * WN];
                         vA[d] = L[(DIM + d) * WN];
-                        vS[d] = L[(2 * DIM + d) * WN];
+                        vS[d] = VS ? L[(2 * DIM + d) * WN] : R(0);
                     }
                     g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
                 });
@@ -558,7 +560,7 @@ __global__ __launch_bounds__(256) void k_g2p1_tiled(GridGeom g, TileGeom tg, Par
                         for (int d = 0; d < DIM; ++d) {
                             vT[d] = gp.vsT[d][n];
                             vA[d] = gp.as[d][n];
-                            vS[d] = vsf[d][n];
+                            vS[d] = VS ? vsf[d][n] : R(0);
                         }
                         g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
                     }
@@ -570,28 +572,6 @@ __global__ __launch_bounds__(256) void k_g2p1_tiled(GridGeom g, TileGeom tg, Par
             for (int a = 0; a < DIM; ++a) pp.pm[a][i] = pv[a];
         }
     }
-}
-
-// ---------------------------------------------------------------------------
-// Tile segments of the sorted particle order
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_tile_segments(const unsigned* __restrict__ keys, int np,
-                                                       int shift, int* __restrict__ seg_begin,
-                                                       int* __restrict__ seg_end,
-                                                       unsigned char* __restrict__ nonempty) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    const unsigned t = keys[i] >> shift;
-    if (i == 0 || (keys[i - 1] >> shift) != t) {
-        seg_begin[t] = i;
-        nonempty[t] = 1;
-    }
-    if (i == np - 1 || (keys[i + 1] >> shift) != t) seg_end[t] = i + 1;
-}
-
-__global__ __launch_bounds__(256) void k_fill_u8(unsigned char* __restrict__ p, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
 }
 
 }  // namespace mpm
