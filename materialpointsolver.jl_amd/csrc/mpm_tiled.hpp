@@ -15,15 +15,17 @@
 // Scatter rounds.  A round is at most 64 particles of one rank of one tile,
 // so at the last sort their bases were pairwise distinct; when a rank group
 // leaves lanes free, the round continues with the next group, which adds into
-// a second window (kScatterWindows), so half-empty groups share a wavefront.  Each lane claims
-// its current base in a small LDS table (write a unique tag, read it back) to
-// confirm that this still holds; the winners' targets are then distinct at
-// every stencil slot and they add into the window with plain LDS
-// read-add-write (volatile, so the slots stay in program order).  LDS float
-// atomics would serialise per lane on this hardware.  A particle that lost its
-// claim or drifted out of the window scatters straight to the grid with
-// global atomics (and a gather reads such a particle's stencil from the grid),
-// so results never depend on the sort interval.
+// a second window (kScatterWindows), so half-empty groups share a wavefront.
+// Each lane claims its current base in a small LDS table (write a unique tag,
+// read it back) to confirm that the bases are still distinct; the winners'
+// targets are then distinct at every stencil slot and they add into the
+// window with plain LDS read-add-write (volatile, so the slots stay in program
+// order).  Losers (particles that drifted onto a base another lane holds)
+// claim again among themselves and add in a second pass.  LDS float atomics
+// would serialise per lane on this hardware.  A particle that lost both claims
+// or drifted out of the window scatters straight to the grid with global
+// atomics (and a gather reads such a particle's stencil from the grid), so
+// results never depend on the sort interval.
 //
 // Tile entries are dealt to the 8 XCDs in contiguous chunks (a workgroup's
 // XCD is its blockIdx modulo 8), so neighbouring tiles, which share halo
@@ -358,27 +360,42 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     if (inside) claim[bidx] = tag;
                     const bool won = inside && claim[bidx] == tag;
                     tag += 64u;
-                    if (won && MODE == 0 && DIM == 3) {
-                        if constexpr (MODE == 0 && DIM == 3)
+                    // A particle that drifted onto a base another lane of the round
+                    // holds claims again among the losers and adds in a second pass.
+                    bool won2 = false;
+                    if (inside && !won) {
+                        claim[bidx] = tag;
+                        won2 = claim[bidx] == tag;
+                    }
+                    tag += 64u;
+                    auto lds_pass = [&] {
+                        if constexpr (MODE == 0 && DIM == 3) {
                             p2g3_window<S, R>(st, m, vol, pv, P.sg, G, win + wb, WW, ZS, WN);
-                    } else if (won) {
-                        for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R dNx, R dNy, R dNz) {
-                            if (N != R(0)) {
-                                R c[NF];
-                                contrib(N, dNx, dNy, dNz, c);
-                                lds_ptr<R> L = win + wb + off;
-                                R old[NF];
+                        } else {
+                            for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R dNx, R dNy, R dNz) {
+                                if (N != R(0)) {
+                                    R c[NF];
+                                    contrib(N, dNx, dNy, dNz, c);
+                                    lds_ptr<R> L = win + wb + off;
+                                    R old[NF];
 #pragma unroll
-                                for (int f = 0; f < NF; ++f) old[f] = L[f * WN];
+                                    for (int f = 0; f < NF; ++f) old[f] = L[f * WN];
 #pragma unroll
-                                for (int f = 0; f < NF; ++f) L[f * WN] = old[f] + c[f];
-                            }
-                            // keep each slot's arithmetic next to its LDS update
-                            __builtin_amdgcn_sched_barrier(0);
-                        });
-                    } else {
-                        // drifted out of the window, or shares its base with another
-                        // particle of the round
+                                    for (int f = 0; f < NF; ++f) L[f * WN] = old[f] + c[f];
+                                }
+                                // keep each slot's arithmetic next to its LDS update
+                                __builtin_amdgcn_sched_barrier(0);
+                            });
+                        }
+                    };
+                    // the two passes in order: the ballot is wave-wide, so the passes
+                    // cannot be merged into one (their lanes may share bases)
+                    if (won) lds_pass();
+                    if (__ballot(won2) != 0ull) {
+                        if (won2) lds_pass();
+                    }
+                    if (!won && !won2) {
+                        // drifted out of the window, or lost both claims
                         for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R dNx, R dNy, R dNz) {
                             if (N != R(0)) {
                                 R c[NF];
