@@ -65,6 +65,11 @@ def parse():
     ap.add_argument("--ens-seq-steps", type=int, default=40)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-return leg")
     ap.add_argument("--sur-steps", type=int, default=200)
+    ap.add_argument("--no-slope", action="store_true",
+                    help="skip the 20M-particle GRF slope leg (exact and learned return)")
+    ap.add_argument("--slope-h", type=float, default=0.5, help="slope grid spacing (0.5 = 20M particles)")
+    ap.add_argument("--slope-k0", type=int, default=400, help="damped K0 steps before the runout")
+    ap.add_argument("--slope-steps", type=int, default=200, help="timed runout steps per form")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -323,6 +328,8 @@ def main():
         out["batched"] = ensemble_leg(a, mpm)
     if not a.no_surrogate:
         out["surrogate"] = surrogate_leg(a, mpm)
+    if not a.no_slope:
+        out["slope"] = slope_leg(a, mpm)
     print(json.dumps(out), flush=True)
 
 
@@ -474,6 +481,88 @@ def surrogate_leg(a, mpm):
           f"{out['learned_particle_steps_per_s']:.3e} particle-steps/s, fallback "
           f"{out['fallback_rate']}, runout diff {out['runout_rel_diff']:+.2e}, MLP kernel "
           f"{tflops} TFLOP/s", file=sys.stderr, flush=True)
+    return out
+
+
+def slope_leg(a, mpm):
+    """The seeded GRF slope at its configured size (20M particles at h = 0.5 m,
+    one Mohr-Coulomb material per initial cell): a damped K0 phase, then the
+    undamped runout timed from that K0 state with the exact 3D Mohr-Coulomb
+    return and with the learned return (trained on return samples logged on the
+    slope itself), each in a fresh context resumed from the K0 state, with
+    captured step graphs."""
+    import numpy as np
+
+    t_setup = time.perf_counter()
+    args, grid, mp, attr, bc, lay = mpm.scenes.slope_3d(h=a.slope_h, seed=1, k0_steps=a.slope_k0)
+    dt = lay["dt"]
+    n = int(mp.np)
+    K = a.full_sort_interval
+    print(f"# slope: {n} particles, {lay['nmat']} materials, {int(grid.ni)} nodes "
+          f"(setup {time.perf_counter() - t_setup:.1f}s)", file=sys.stderr, flush=True)
+    m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=K)
+    try:
+        t0 = time.perf_counter()
+        Ti = m.run(m.step_args(args), dt, 0.0, lay["k0_steps"], graph=True)
+        m.sync()
+        t_k0 = time.perf_counter() - t0
+        k0 = m.download()
+        ra = m.step_args(lay["runout_args"])
+        m.return_log(2_000_000, 8)
+        m.run(ra, dt, Ti, 60)
+        X, Y = m.read_return_log()
+    finally:
+        m.close()
+    print(f"# slope: K0 {lay['k0_steps']} steps in {t_k0:.2f}s, {len(X)} return samples",
+          file=sys.stderr, flush=True)
+    snap = dict(coords=k0["xi"], velocity_s=k0["vs"], stress=k0["sigma"], strain_s=k0["eps_s"],
+                eqstrain=k0["eps_q"], ekstrain=k0["eps_k"], volume=k0["omega"], mass_s=k0["ms"],
+                time=np.asarray(Ti))
+    S = None
+    if len(X) >= 1000:
+        S = mpm.Surrogate(batch_capacity=4096, seed=0)
+        S.fit_normalisation(X, Y)
+        hist = S.train(X, Y, steps=2000, batch=4096, lr=2e-3, seed=1)
+        print(f"# slope: trained on {len(X)} samples, loss {hist[:20].mean():.4f} -> "
+              f"{hist[-20:].mean():.5f}", file=sys.stderr, flush=True)
+    res = {}
+    for mode in ("exact", "learned"):
+        if mode == "learned" and S is None:
+            continue
+        m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=K)
+        try:
+            T = m.upload_snapshot(snap, mp.Ω0)
+            if mode == "learned":
+                m.set_surrogate(S, tol=1e-3)
+            T = m.run(ra, dt, T, 20, graph=True)
+            m.sync()
+            t0 = time.perf_counter()
+            m.run(ra, dt, T, a.slope_steps, graph=True)
+            m.sync()
+            t = time.perf_counter() - t0
+            res[mode] = (t, m.download()["xi"], m.surrogate_stats() if mode == "learned" else (0, 0))
+        finally:
+            m.close()
+        print(f"# slope: {mode} {a.slope_steps} runout steps in {t:.2f}s, "
+              f"{n * a.slope_steps / t:.3e} particle-steps/s", file=sys.stderr, flush=True)
+    if S is not None:
+        S.close()
+    out = {"workload": f"GRF slope, {n} particles (h = {a.slope_h} m, 8 per cell), "
+                       f"{lay['nmat']} Mohr-Coulomb materials, {int(grid.ni)} grid nodes, FP32, "
+                       f"K0 {lay['k0_steps']} steps, then {a.slope_steps} timed runout steps "
+                       f"after 20, sort every {K}, captured graphs",
+           "particles": n,
+           "k0_seconds": t_k0,
+           "exact_particle_steps_per_s": n * a.slope_steps / res["exact"][0]}
+    if "learned" in res:
+        plastic, fb = res["learned"][2]
+        xe, xl = res["exact"][1], res["learned"][1]
+        out.update({"learned_particle_steps_per_s": n * a.slope_steps / res["learned"][0],
+                    "speed_ratio_learned_over_exact": res["exact"][0] / res["learned"][0],
+                    "fallback_rate": fb / plastic if plastic else None,
+                    "displacement_rel_l2": float(np.linalg.norm(xl - xe) /
+                                                 max(np.linalg.norm(xe - k0["xi"]), 1e-300)),
+                    "samples": int(len(X))})
     return out
 
 

@@ -1,6 +1,6 @@
 """Autotuner for the step's runtime knobs: the sort interval and whether the
 LDS tile kernels are used (the tile shape and window size are compile-time:
-4x4x4 bases in 3D, 8x8 in 2D, one base of slack per side).
+4x4x4 bases in 3D, 8x8 in 2D, windows without slack, two per scatter wavefront).
 
 ``autotune`` times candidate settings on the live model, in place: each
 candidate runs ``warm`` untimed steps and ``steps`` timed steps of the actual
