@@ -105,6 +105,14 @@ def test_fp64_3d_schemes(mpm, orc, scheme, constitutive, sort):
     _compare(mpm, orc, sc, 100, 1e-9, sort_interval=sort)
 
 
+@pytest.mark.parametrize("scheme", ["USL", "USF"])
+def test_fp64_schemes_linear_basis(mpm, orc, scheme):
+    # the linear basis (NIC 4 / 8) under both schedules, plain and tiled kernels
+    for sort in (0, 5):
+        _compare(mpm, orc, _c2(mpm, basis="linear", scheme=scheme), 100, 1e-9, sort_interval=sort)
+        _compare(mpm, orc, _c3(mpm, basis="linear", scheme=scheme), 60, 1e-9, sort_interval=sort)
+
+
 def test_fp64_schemes_mvl_and_ramp(mpm, orc):
     # USF runs MVL after the position update, on the stress of the step's head
     for scheme in ("USL", "USF"):
