@@ -152,12 +152,17 @@ int mpm_run(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int64_t ns
 int mpm_run_graph(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int64_t nsteps,
                   double *Ti_out);
 
-/* Runtime options of a context (no reference counterpart; for the autotuner):
+/* Runtime options of a context (no reference counterpart; for the autotuner
+ * and the benchmark):
  *  MPM_OPT_SORT_INTERVAL: steps between sorts, >= 1; only for contexts created
  *                         with sort_interval > 0 (the sort buffers exist).
  *  MPM_OPT_TILES:         1 = LDS tile kernels after a sort (default), 0 = the
- *                         plain global-atomic kernels. */
-enum { MPM_OPT_SORT_INTERVAL = 0, MPM_OPT_TILES = 1 };
+ *                         plain global-atomic kernels.
+ *  MPM_OPT_PROFILE_MASK:  bit k set = mpm_profile_enable times kernel k of
+ *                         mpm_profile_kernel_name's order; 0 = every kernel
+ *                         (default).  Fewer event pairs per step keep the
+ *                         profiled steps closer to unprofiled ones. */
+enum { MPM_OPT_SORT_INTERVAL = 0, MPM_OPT_TILES = 1, MPM_OPT_PROFILE_MASK = 2 };
 int mpm_set_option(mpm_ctx *ctx, int option, int64_t value);
 
 /* Wait for all work on the context's stream; reports range/finite flags the

@@ -22,7 +22,7 @@ MPM_UGIMP, MPM_LINEAR = 0, 1
 MPM_MUSL, MPM_USL, MPM_USF = 0, 1, 2
 SCHEME = {"MUSL": MPM_MUSL, "USL": MPM_USL, "USF": MPM_USF}
 MPM_VA_A, MPM_VA_V = 0, 1
-MPM_OPT_SORT_INTERVAL, MPM_OPT_TILES = 0, 1
+MPM_OPT_SORT_INTERVAL, MPM_OPT_TILES, MPM_OPT_PROFILE_MASK = 0, 1, 2
 CONSTITUTIVE = {"linearelastic": 0, "druckerprager": 1, "mohrcoulomb": 2}
 
 

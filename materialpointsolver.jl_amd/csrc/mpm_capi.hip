@@ -64,6 +64,7 @@ struct mpm_ctx {
     long long steps = 0;
     bool resort = true;  // particle order changed outside the sort (upload)
     bool prof_on = false;
+    uint64_t prof_mask = 0;  // kernel ids to time; 0 = all
     bool use_tiles = true;  // LDS-binned scatters when the particles are tile-sorted
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> pool;
@@ -104,14 +105,15 @@ struct mpm_ctx {
     template <typename Fn>
     int launch(int kid, Fn&& fn) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (prof_on) {
+        const bool rec = prof_on && (prof_mask == 0 || ((prof_mask >> kid) & 1ull));
+        if (rec) {
             e0 = event();
             if (e0) hipEventRecord(e0, stream);
         }
         fn();
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, kKernelNames[kid]);
-        if (prof_on && e0) {
+        if (rec && e0) {
             e1 = event();
             if (e1) {
                 hipEventRecord(e1, stream);
@@ -1122,6 +1124,11 @@ int mpm_set_option(mpm_ctx* c, int option, int64_t value) {
         c->cfg.sort_interval = int32_t(value);
     } else if (option == MPM_OPT_TILES) {
         c->use_tiles = value != 0;
+    } else if (option == MPM_OPT_PROFILE_MASK) {
+        if (value < 0 || (kNumKernels < 63 && value >= (int64_t(1) << kNumKernels)))
+            return c->fail(MPM_E_ARG, "mpm_set_option: profile mask names no such kernel");
+        c->prof_mask = uint64_t(value);
+        return MPM_OK;
     } else {
         return c->fail(MPM_E_ARG, "mpm_set_option: unknown option");
     }

@@ -201,7 +201,19 @@ class HIPModel:
         return int(out[0])
 
     def set_option(self, option, value):
-        """Runtime option: "sort_interval" (>= 1) or "tiles" (bool)."""
+        """Runtime option: "sort_interval" (>= 1), "tiles" (bool) or
+        "profile_kernels" (kernel names that profile_enable times; empty or
+        None = every kernel)."""
+        if option == "profile_kernels":
+            names = [self.lib.mpm_profile_kernel_name(k).decode()
+                     for k in range(self.lib.mpm_profile_kernel_count())]
+            mask = 0
+            for name in value or ():
+                if name not in names:
+                    raise ValueError(f"unknown kernel {name!r}; kernels: {names}")
+                mask |= 1 << names.index(name)
+            self._check(self.lib.mpm_set_option(self.ctx, capi.MPM_OPT_PROFILE_MASK, mask))
+            return
         code = {"sort_interval": capi.MPM_OPT_SORT_INTERVAL, "tiles": capi.MPM_OPT_TILES}[option]
         self._check(self.lib.mpm_set_option(self.ctx, code, int(value)))
 

@@ -187,3 +187,38 @@ def test_resume_from_snapshot(mpm, tmp_path, eps):
         r = _rel(got[k] - (x0 if k == "xi" else 0), ref[k] - (x0 if k == "xi" else 0))
         print(f"  {k}: rel L2 {r:.1e}", flush=True)
         assert r <= tol, k
+
+
+def test_profile_mask_times_only_the_named_kernels(mpm):
+    # MPM_OPT_PROFILE_MASK: events around the masked kernels only; 0 = all
+    sc = mpm.scenes.column_collapse_3d(h=0.02, width=0.16, height=0.08, box_xy=0.4, box_z=0.12,
+                                       nsteps=20)
+    args, grid, mp, attr, bc = sc
+    dt = float(args.ΔT)
+    m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=4)
+    try:
+        sa = m.step_args(args)
+        m.profile_enable(True)
+        T = m.run(sa, dt, 0.0, 8)
+        every = m.profile_read()
+        m.profile_enable(False)
+        assert every["k_p2g"][1] == 8 and every["k_grid1"][1] == 8 and every["k_permute"][1] == 2
+        assert every["k_sort"][1] > 0
+        m.set_option("profile_kernels", ["k_p2g", "k_sort"])
+        m.profile_enable(True)
+        T = m.run(sa, dt, T, 8)
+        some = m.profile_read()
+        m.profile_enable(False)
+        assert some["k_p2g"][1] == 8 and some["k_sort"][1] == every["k_sort"][1]
+        assert some["k_p2g"][0] > 0.0
+        assert all(c == 0 for k, (_, c) in some.items() if k not in ("k_p2g", "k_sort"))
+        with pytest.raises(ValueError):
+            m.set_option("profile_kernels", ["no_such_kernel"])
+        m.set_option("profile_kernels", None)
+        m.profile_enable(True)
+        m.run(sa, dt, T, 4)
+        again = m.profile_read()
+        m.profile_enable(False)
+        assert again["k_grid2"][1] == 4
+    finally:
+        m.close()
