@@ -48,7 +48,8 @@ def parse():
     ap.add_argument("--precision", default="FP32", choices=["FP32", "FP64"])
     ap.add_argument("--sort-interval", type=int, default=10)
     ap.add_argument("--h", type=float, default=0.005, help="grid spacing (0.005 = 2,048,000 particles)")
-    ap.add_argument("--cpu-steps", type=int, default=6, help="oracle steps for the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=40,
+                    help="oracle steps for the CPU baseline (about 12 s on 16 cores)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(cores, 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
