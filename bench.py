@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--ens-seq-steps", type=int, default=40)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-return leg")
     ap.add_argument("--sur-steps", type=int, default=200)
+    ap.add_argument("--no-schedules", action="store_true",
+                    help="skip the USL/USF schedule leg (same workload under the other schedules)")
     ap.add_argument("--no-slope", action="store_true",
                     help="skip the 20M-particle GRF slope leg (exact and learned return)")
     ap.add_argument("--slope-h", type=float, default=0.5, help="slope grid spacing (0.5 = 20M particles)")
@@ -357,6 +359,8 @@ def main():
         out["batched"] = ensemble_leg(a, mpm)
     if not a.no_surrogate:
         out["surrogate"] = surrogate_leg(a, mpm)
+    if not a.no_schedules:
+        out["schedules"] = schedules_leg(a, mpm, value)
     if not a.no_slope:
         out["slope"] = slope_leg(a, mpm)
     print(json.dumps(out), flush=True)
@@ -428,6 +432,39 @@ def ensemble_leg(a, mpm):
           f"instance-steps/s; sequential {seq['eager']:.0f} / {seq['graph']:.0f}", file=sys.stderr,
           flush=True)
     return out
+
+
+def schedules_leg(a, mpm, musl_value):
+    """The timed workload under the USL and USF schedules (src/solvers/OS_USL.jl,
+    OS_USF.jl): same particles, sort interval, warm-up and step count as the
+    MUSL line, no events."""
+    import numpy as np
+
+    res = {"MUSL": {"particle_steps_per_s": musl_value}}
+    for scheme in ("USL", "USF"):
+        args, grid, mp, attr, bc = mpm.scenes.column_collapse_3d(h=a.h, eps=a.precision,
+                                                                  scheme=scheme)
+        dt = float(args.ΔT)
+        m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=a.sort_interval)
+        try:
+            sa = m.step_args(args)
+            Ti = m.run(sa, dt, 0.0, a.warmup)
+            m.sync()
+            t0 = time.perf_counter()
+            m.run(sa, dt, Ti, a.steps)
+            m.sync()
+            el = time.perf_counter() - t0
+            st = m.download()
+        finally:
+            m.close()
+        n = int(mp.np)
+        finite = bool(np.isfinite(st["xi"]).all() and np.isfinite(st["sigma"]).all())
+        res[scheme] = {"particle_steps_per_s": n * a.steps / el, "ms_per_step": el / a.steps * 1e3,
+                       "finite": finite}
+        print(f"# schedule {scheme}: {res[scheme]['ms_per_step']:.4f} ms/step, "
+              f"{res[scheme]['particle_steps_per_s']:.3e} particle-steps/s", file=sys.stderr,
+              flush=True)
+    return res
 
 
 def surrogate_leg(a, mpm):
