@@ -231,3 +231,35 @@ def test_fp64_body_force_extension(mpm, orc):
     sc2 = _c2(mpm)
     sc2[0].ext = {"gravity_x": 3.0}
     _compare(mpm, orc, sc2, 100, 1e-9)
+
+
+def test_fp32_tiled_kernels_match_plain_kernels(mpm):
+    # A dense FP32 column right after the sort, where most rounds of the tile
+    # scatters are full (64 particles on the 64 bases of their tile): the LDS
+    # tile kernels and the plain global-atomic kernels sum the same terms, so the
+    # two differ by FP32 summation order only (a missing or misplaced term would
+    # show at 1e-2 and above).  Stresses come from differences of nearly equal
+    # nodal velocities, which amplifies that rounding; the bounds leave room.
+    args, grid, mp, attr, bc = mpm.scenes.column_collapse_3d(
+        h=0.01, width=0.16, height=0.08, box_xy=0.4, box_z=0.12, eps="FP32", nsteps=10)
+    dt = float(args.ΔT)
+    x0 = np.asarray(mp.ξ, dtype=np.float64)
+    tiled = mpm.HIPModel(grid, mp, attr, bc, sort_interval=10)
+    plain = mpm.HIPModel(grid, mp, attr, bc, sort_interval=10)
+    try:
+        plain.set_option("tiles", False)
+        sa = tiled.step_args(args)
+        T = 0.0
+        for nsteps, tol in ((1, 1e-4), (4, 1e-3)):
+            tiled.run(sa, dt, T, nsteps)
+            T = plain.run(sa, dt, T, nsteps)
+            a, b = tiled.download(), plain.download()
+            for k in ("xi", "vs", "sigma"):
+                ga = a[k] - x0 if k == "xi" else a[k]
+                gb = b[k] - x0 if k == "xi" else b[k]
+                r = _rel(ga, gb)
+                print(f"  after {nsteps} more step(s): {k} rel L2 {r:.2e}", flush=True)
+                assert r <= tol, (k, nsteps, r)
+    finally:
+        tiled.close()
+        plain.close()
