@@ -209,8 +209,8 @@ def main():
         pmc_child(a)
         return
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md but not built.", file=sys.stderr)
+        print(f"bench.py: the solver runs on one GPU only (asked for --gpus {a.gpus}); "
+              "spatial-slab sharding across GPUs is not built.", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import __graft_entry__ as ge
