@@ -70,10 +70,16 @@ class HIPModel:
     def _upload_particles(self, arrs):
         if arrs["nid"].size != self.np:
             raise ValueError("attr.nid must have one entry per particle")
+        nv = 4 if self.dim == 2 else 6
+        cols = dict(xi=self.dim, vs=self.dim, sigma=nv, eps_s=nv, F=self.dim * self.dim)
         for k in ("xi", "vs", "sigma", "eps_s", "F", "omega", "omega0", "rho", "rho0", "eps_q",
                   "eps_k"):
             if arrs[k].shape[0] != self.np:
                 raise ValueError(f"{k} must have np = {self.np} rows")
+            want = cols.get(k, 1)
+            got = arrs[k].shape[1] if arrs[k].ndim == 2 else 1
+            if got != want or arrs[k].size != self.np * want:
+                raise ValueError(f"{k} must be np x {want} (got shape {arrs[k].shape})")
         d = capi.dptr
         self._check(self.lib.mpm_upload_particles(
             self.ctx, d(arrs["xi"]), d(arrs["vs"]), d(arrs["sigma"]), d(arrs["eps_s"]),

@@ -195,3 +195,15 @@ def test_snapshot_state_for_resume(mpm):
                        ("eps_s", "strain_s"), ("eps_q", "eqstrain"), ("eps_k", "ekstrain")):
             np.testing.assert_array_equal(arrs[k], g[src])
             assert arrs[k].flags.f_contiguous
+
+
+def test_particle_arrays_with_wrong_columns_are_rejected(mpm, monkeypatch):
+    # the C ABI reads np x k host arrays; a wrong column count must stop on the host
+    rec = _Recorder()
+    monkeypatch.setattr(mpm._capi, "_lib", rec)
+    args, grid, mp, attr, bc = mpm.scenes.column_collapse_3d(h=0.04, width=0.16, height=0.08,
+                                                             box_xy=0.4, box_z=0.16, nsteps=2)
+    mp.σij = np.asfortranarray(mp.σij[:, :4])
+    with pytest.raises(ValueError, match="sigma must be np x 6"):
+        mpm.HIPModel(grid, mp, attr, bc)
+    assert "mpm_upload_particles" not in [c[0] for c in rec.calls]
