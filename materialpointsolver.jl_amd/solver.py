@@ -135,6 +135,8 @@ class HIPModel:
         self._upload_particles(arrs)
         self._nid = arrs["nid"]
         m = [f64(a) for a in (attr.Ks, attr.Gs, attr.ϕ, attr.ψ, attr.c, attr.σt, attr.Hp, attr.cr)]
+        if any(a.size != int(attr.nmat) for a in m):
+            raise ValueError(f"material arrays must have attr.nmat = {int(attr.nmat)} entries")
         self._check(self.lib.mpm_upload_materials(self.ctx, *[capi.dptr(a) for a in m]))
         ni = int(grid.ni)
         bx = [i64(bc.vx_s_idx), f64(bc.vx_s_val), i64(bc.vy_s_idx), f64(bc.vy_s_val)]
