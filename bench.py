@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--ens-seq-steps", type=int, default=40)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-return leg")
     ap.add_argument("--sur-steps", type=int, default=200)
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the FP64 / volumetric-locking variants of the timed workload")
     ap.add_argument("--no-schedules", action="store_true",
                     help="skip the USL/USF schedule leg (same workload under the other schedules)")
     ap.add_argument("--no-slope", action="store_true",
@@ -359,6 +361,8 @@ def main():
         out["batched"] = ensemble_leg(a, mpm)
     if not a.no_surrogate:
         out["surrogate"] = surrogate_leg(a, mpm)
+    if not a.no_variants:
+        out["variants"] = variants_leg(a, mpm)
     if not a.no_schedules:
         out["schedules"] = schedules_leg(a, mpm, value)
     if not a.no_slope:
@@ -432,6 +436,39 @@ def ensemble_leg(a, mpm):
           f"instance-steps/s; sequential {seq['eager']:.0f} / {seq['graph']:.0f}", file=sys.stderr,
           flush=True)
     return out
+
+
+def _timed_run(a, mpm, **scene):
+    """Warm-up + timed steps of the 3D column with the given scene options, no
+    events; returns (particle-steps/s, ms/step)."""
+    args, grid, mp, attr, bc = mpm.scenes.column_collapse_3d(h=a.h, **scene)
+    dt = float(args.ΔT)
+    m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=a.sort_interval)
+    try:
+        sa = m.step_args(args)
+        Ti = m.run(sa, dt, 0.0, a.warmup)
+        m.sync()
+        t0 = time.perf_counter()
+        m.run(sa, dt, Ti, a.steps)
+        m.sync()
+        el = time.perf_counter() - t0
+    finally:
+        m.close()
+    return int(mp.np) * a.steps / el, el / a.steps * 1e3
+
+
+def variants_leg(a, mpm):
+    """The timed workload in the FP64 parity mode and with volumetric locking
+    (MVL, utils_OS.jl:1162-1256) on: the settings of the reference's own 3D
+    collapse notebook run (FP64, MVL on)."""
+    res = {}
+    for name, kw in (("fp64", dict(eps="FP64")), ("fp64_mvl", dict(eps="FP64", MVL=True)),
+                     ("fp32_mvl", dict(eps="FP32", MVL=True))):
+        v, ms = _timed_run(a, mpm, **kw)
+        res[name] = {"particle_steps_per_s": v, "ms_per_step": ms}
+        print(f"# variant {name}: {ms:.4f} ms/step, {v:.3e} particle-steps/s", file=sys.stderr,
+              flush=True)
+    return res
 
 
 def schedules_leg(a, mpm, musl_value):
