@@ -73,3 +73,16 @@ def test_null_arguments(mpm):
     assert lib.mpm_sync(None) == -1
     lib.mpm_destroy(None)  # no-op
     assert lib.mpm_tpeak(10, 1, None) == -1
+
+
+def test_python_constants_match_the_header(mpm):
+    # every MPM_* constant the ctypes layer defines carries the header's value
+    src = open(os.path.join(ROOT, "include", "mpm_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    vals = {n: int(v) for n, v in re.findall(r"\b(MPM_[A-Z0-9_]+)\s*=\s*(-?\d+)", src)}
+    vals.update({n: int(v) for n, v in re.findall(r"#define\s+(MPM_[A-Z0-9_]+)\s+(-?\d+)", src)})
+    capi = mpm._capi
+    shared = [n for n in vals if hasattr(capi, n)]
+    assert {"MPM_OPT_PROFILE_MASK", "MPM_USF", "MPM_OK"} <= set(shared)
+    for n in shared:
+        assert getattr(capi, n) == vals[n], n
