@@ -86,3 +86,7 @@ def test_python_constants_match_the_header(mpm):
     assert {"MPM_OPT_PROFILE_MASK", "MPM_USF", "MPM_OK"} <= set(shared)
     for n in shared:
         assert getattr(capi, n) == vals[n], n
+    # the error-code names and the constitutive ids the host layer maps
+    assert {vals[n]: n for n in capi.ERRORS.values()} == capi.ERRORS
+    for name, code in capi.CONSTITUTIVE.items():
+        assert vals["MPM_" + name.upper()] == code, name
