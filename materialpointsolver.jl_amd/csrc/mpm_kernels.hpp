@@ -894,6 +894,29 @@ __device__ __forceinline__ void g2p2_commit(const GridGeom& g, PartPtrs<R>& pp, 
                                             const Material<R>* __restrict__ mats,
                                             const StepArgs<R>& sa, int i, const double* x, R* F) {
     constexpr int NS = DIM == 2 ? 4 : 6;
+    // Every load of the commit first: the particle streams are separate arrays,
+    // but the compiler cannot know they do not alias, so a load placed after a
+    // store waits for it; loading up front lets them all be in flight at once.
+    R eps0[NS], s[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) eps0[k] = (DIM == 2 && k == 2) ? R(0) : pp.eps[k][i];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s[k] = pp.sig[k][i];
+    const double J0 = pp.J[i];
+    const R vol0 = pp.vol0[i];
+    const int nid = pp.nid[i];
+    R eq = R(0), ek = R(0);
+    if (CM != 0 && sa.plastic) {
+        eq = pp.eq[i];
+        ek = pp.ek[i];
+    }
+    R dxp[3] = {R(0), R(0), R(0)};
+    if (sa.commit_dx) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) dxp[d] = pp.dxp[d][i];
+    }
+    const Material<R> M = mats[nid];
+
     const R rdt = R(1) / sa.dt;
     R e[NS];
     double detI;
@@ -904,9 +927,9 @@ __device__ __forceinline__ void g2p2_commit(const GridGeom& g, PartPtrs<R>& pp, 
         e[1] = F[3];
         e[2] = R(0);
         e[3] = F[1] + F[2];
-        pp.eps[0][i] += e[0];
-        pp.eps[1][i] += e[1];
-        pp.eps[3][i] += e[3];
+        pp.eps[0][i] = eps0[0] + e[0];
+        pp.eps[1][i] = eps0[1] + e[1];
+        pp.eps[3][i] = eps0[3] + e[3];
         double a = 1.0 + double(F[0]), b = double(F[1]), c = double(F[2]), d = 1.0 + double(F[3]);
         detI = a * d - b * c;
     } else {
@@ -922,7 +945,7 @@ __device__ __forceinline__ void g2p2_commit(const GridGeom& g, PartPtrs<R>& pp, 
         e[4] = F[5] + F[7];
         e[5] = F[2] + F[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) pp.eps[k][i] += e[k];
+        for (int k = 0; k < 6; ++k) pp.eps[k][i] = eps0[k] + e[k];
         double A[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) A[k] = double(F[k]);
@@ -932,18 +955,13 @@ __device__ __forceinline__ void g2p2_commit(const GridGeom& g, PartPtrs<R>& pp, 
         detI = A[0] * A[4] * A[8] + A[1] * A[5] * A[6] + A[2] * A[3] * A[7] -
                A[6] * A[4] * A[2] - A[7] * A[5] * A[0] - A[8] * A[3] * A[1];
     }
-    const double J = pp.J[i] * detI;
+    const double J = J0 * detI;
     pp.J[i] = J;
-    const R vol = R(J * double(pp.vol0[i]));
+    const R vol = R(J * double(vol0));
     pp.vol[i] = vol;
 
-    const Material<R> M = mats[pp.nid[i]];
-    R s[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) s[k] = pp.sig[k][i];
     liE<DIM, R>(s, F, e, M);
     if (CM != 0 && sa.plastic) {
-        R eq = pp.eq[i], ek = pp.ek[i];
         if constexpr (CM == 1) {
             dpP<DIM, R>(s, eq, ek, M);
         } else if constexpr (DIM == 2) {
@@ -979,7 +997,7 @@ __device__ __forceinline__ void g2p2_commit(const GridGeom& g, PartPtrs<R>& pp, 
     }
     if (sa.commit_dx) {
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) pp.x[d][i] = x[d] + double(pp.dxp[d][i]);
+        for (int d = 0; d < DIM; ++d) pp.x[d][i] = x[d] + double(dxp[d]);
     }
 }
 
