@@ -480,12 +480,15 @@ __device__ __forceinline__ void g2p1_accumulate(int va, R N, const R* vT, const 
 template <int DIM, typename R>
 __device__ __forceinline__ void g2p1_commit(PartPtrs<R>& pp, const StepArgs<R>& sa, int i,
                                             const R* A, const R* B, const R* C, R* pv) {
+    // loads first: a load after a store to another stream would wait for it
     const R m = pp.ms[i];
+    R v[3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) v[d] = pp.v[d][i];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        R v = pp.v[d][i];
-        R vn = (sa.va == 0) ? sa.flip * v + sa.pic * C[d] + B[d] * sa.dt
-                            : sa.flip * (v + B[d]) + sa.pic * C[d];
+        R vn = (sa.va == 0) ? sa.flip * v[d] + sa.pic * C[d] + B[d] * sa.dt
+                            : sa.flip * (v[d] + B[d]) + sa.pic * C[d];
         pp.v[d][i] = vn;
         pp.dxp[d][i] = A[d] * sa.dt;
         pv[d] = m * vn;
