@@ -408,11 +408,14 @@ __global__ __launch_bounds__(256) void k_grid1(GridGeom g, GridPtrs<R> gp, StepA
         ss += fd * fd;
     }
     const R damp = -sa.zeta * sqrt(ss);
+    R bv[3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) bv[d] = (bc & (1u << d)) ? gp.bcv[d][n] : R(0);
     R vT[3];
     if (sa.va == 0) {
 #pragma unroll
         for (int d = 0; d < DIM; ++d)
-            if (bc & (1u << d)) P[d] = gp.bcv[d][n];
+            if (bc & (1u << d)) P[d] = bv[d];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
             vs[d] = P[d] * msd;
@@ -420,8 +423,8 @@ __global__ __launch_bounds__(256) void k_grid1(GridGeom g, GridPtrs<R> gp, StepA
             vT[d] = vs[d] + Fs * sa.dt * msd;
             R ac = Fs * msd;
             if (bc & (1u << d)) {
-                vT[d] = gp.bcv[d][n];
-                ac = gp.bcv[d][n];
+                vT[d] = bv[d];
+                ac = bv[d];
             }
             gp.vs[d][n] = vs[d];
             gp.vsT[d][n] = vT[d];
@@ -433,7 +436,7 @@ __global__ __launch_bounds__(256) void k_grid1(GridGeom g, GridPtrs<R> gp, StepA
             vs[d] = P[d] * msd;
             R Fs = Fc[d] + damp * sgn(vs[d]);
             vT[d] = (P[d] + Fs * sa.dt) * msd;
-            if (bc & (1u << d)) vT[d] = gp.bcv[d][n];
+            if (bc & (1u << d)) vT[d] = bv[d];
             gp.vs[d][n] = vs[d];
             gp.vsT[d][n] = vT[d];
         }
@@ -552,10 +555,17 @@ __global__ __launch_bounds__(256) void k_grid2(GridGeom g, GridPtrs<R> gp, R dt)
     if (m == R(0)) return;  // no mass: p is zero and no G2P weight reads du here
     const R msd = m < R(kEpsF64) ? R(0) : R(1) / m;
     const unsigned bc = gp.bcf[n];
+    // loads before stores (a load after a store to another field waits for it)
+    R P[3], bv[3];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        R v = gp.p[d][n] * msd;
-        if (bc & (1u << d)) v = gp.bcv[d][n];
+        P[d] = gp.p[d][n];
+        bv[d] = (bc & (1u << d)) ? gp.bcv[d][n] : R(0);
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        R v = P[d] * msd;
+        if (bc & (1u << d)) v = bv[d];
         gp.du[d][n] = v * dt;
         gp.p[d][n] = R(0);
     }
