@@ -902,33 +902,61 @@ __device__ __forceinline__ int cell_of(const GridGeom& g, const double* x) {
 // vollock1_OS!).  x is the step-start position.  CM is the constitutive
 // model (0 linear elastic, 1 Drucker-Prager, 2 Mohr-Coulomb), a template
 // parameter so that each kernel holds the registers of one model only.
+// The particle state G2P #2's commit reads.  Every load of the commit is
+// issued before any of its stores: the particle streams are separate arrays,
+// but the compiler cannot know they do not alias, so a load placed after a
+// store waits for it.  Callers load it before the gather, so the loads are in
+// flight while the stencil is summed.
+template <int DIM, typename R>
+struct CommitIn {
+    static constexpr int NS = DIM == 2 ? 4 : 6;
+    R eps0[NS], s[NS];
+    double J0;
+    R vol0, eq, ek, dxp[3];
+    int nid;
+};
+
+template <int DIM, typename R, int CM>
+__device__ __forceinline__ void load_commit_in(const PartPtrs<R>& pp, const StepArgs<R>& sa, int i,
+                                               CommitIn<DIM, R>& c) {
+    constexpr int NS = DIM == 2 ? 4 : 6;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) c.eps0[k] = (DIM == 2 && k == 2) ? R(0) : pp.eps[k][i];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) c.s[k] = pp.sig[k][i];
+    c.J0 = pp.J[i];
+    c.vol0 = pp.vol0[i];
+    c.nid = pp.nid[i];
+    c.eq = R(0);
+    c.ek = R(0);
+    if (CM != 0 && sa.plastic) {
+        c.eq = pp.eq[i];
+        c.ek = pp.ek[i];
+    }
+    c.dxp[0] = c.dxp[1] = c.dxp[2] = R(0);
+    if (sa.commit_dx) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) c.dxp[d] = pp.dxp[d][i];
+    }
+}
+
 template <int DIM, typename R, int CM>
 __device__ __forceinline__ void g2p2_commit(const GridGeom& g, PartPtrs<R>& pp, GridPtrs<R>& gp,
                                             const Material<R>* __restrict__ mats,
-                                            const StepArgs<R>& sa, int i, const double* x, R* F) {
+                                            const StepArgs<R>& sa, int i, const double* x, R* F,
+                                            const CommitIn<DIM, R>& cin) {
     constexpr int NS = DIM == 2 ? 4 : 6;
-    // Every load of the commit first: the particle streams are separate arrays,
-    // but the compiler cannot know they do not alias, so a load placed after a
-    // store waits for it; loading up front lets them all be in flight at once.
     R eps0[NS], s[NS];
 #pragma unroll
-    for (int k = 0; k < NS; ++k) eps0[k] = (DIM == 2 && k == 2) ? R(0) : pp.eps[k][i];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) s[k] = pp.sig[k][i];
-    const double J0 = pp.J[i];
-    const R vol0 = pp.vol0[i];
-    const int nid = pp.nid[i];
-    R eq = R(0), ek = R(0);
-    if (CM != 0 && sa.plastic) {
-        eq = pp.eq[i];
-        ek = pp.ek[i];
+    for (int k = 0; k < NS; ++k) {
+        eps0[k] = cin.eps0[k];
+        s[k] = cin.s[k];
     }
-    R dxp[3] = {R(0), R(0), R(0)};
-    if (sa.commit_dx) {
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) dxp[d] = pp.dxp[d][i];
-    }
-    const Material<R> M = mats[nid];
+    const double J0 = cin.J0;
+    const R vol0 = cin.vol0;
+    R eq = cin.eq, ek = cin.ek;
+    const R* dxp = cin.dxp;
+    const Material<R> M = mats[cin.nid];
 
     const R rdt = R(1) / sa.dt;
     R e[NS];
@@ -1028,6 +1056,8 @@ __global__ __launch_bounds__(256) void k_g2p2(GridGeom g, PartPtrs<R> pp, GridPt
     if (i >= np) return;
     double x[3];
     load_x<DIM>(pp.x, i, x);
+    CommitIn<DIM, R> cin;
+    load_commit_in<DIM, R, CM>(pp, sa, i, cin);
     Stencil<DIM, S, R> st;
     int flag = 0;
     if (!make_stencil<DIM, S, R>(g, x, st, flag)) {
@@ -1053,7 +1083,7 @@ __global__ __launch_bounds__(256) void k_g2p2(GridGeom g, PartPtrs<R> pp, GridPt
             }
         }
     });
-    g2p2_commit<DIM, R, CM>(g, pp, gp, mats, sa, i, x, F);
+    g2p2_commit<DIM, R, CM>(g, pp, gp, mats, sa, i, x, F, cin);
     raise_flag(flags, flag);
 }
 
