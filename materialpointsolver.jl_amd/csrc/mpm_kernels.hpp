@@ -482,12 +482,10 @@ __device__ __forceinline__ void g2p1_accumulate(int va, R N, const R* vT, const 
 // New particle velocity and position increment; returns the new momentum.
 template <int DIM, typename R>
 __device__ __forceinline__ void g2p1_commit(PartPtrs<R>& pp, const StepArgs<R>& sa, int i,
-                                            const R* A, const R* B, const R* C, R* pv) {
-    // loads first: a load after a store to another stream would wait for it
-    const R m = pp.ms[i];
-    R v[3];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) v[d] = pp.v[d][i];
+                                            const R* A, const R* B, const R* C, R* pv, R m,
+                                            const R* v) {
+    // m and v are loaded by the caller before the gather: a load after a store
+    // to another stream would wait for it
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         R vn = (sa.va == 0) ? sa.flip * v[d] + sa.pic * C[d] + B[d] * sa.dt
@@ -511,6 +509,10 @@ __global__ __launch_bounds__(256) void k_g2p1_p2g2(GridGeom g, PartPtrs<R> pp, G
     if (i >= np) return;
     double x[3];
     load_x<DIM>(pp.x, i, x);
+    const R m0 = pp.ms[i];
+    R v0[3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) v0[d] = pp.v[d][i];
     Stencil<DIM, S, R> st;
     int flag = 0;
     if (!make_stencil<DIM, S, R>(g, x, st, flag)) {
@@ -534,7 +536,7 @@ __global__ __launch_bounds__(256) void k_g2p1_p2g2(GridGeom g, PartPtrs<R> pp, G
         }
     });
     R pv[3];
-    g2p1_commit<DIM, R>(pp, sa, i, A, B, C, pv);
+    g2p1_commit<DIM, R>(pp, sa, i, A, B, C, pv, m0, v0);
     if (!sa.remap) return;  // USL/USF: no momentum re-map
     for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
         if (N != R(0)) {

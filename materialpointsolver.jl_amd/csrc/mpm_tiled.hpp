@@ -545,6 +545,10 @@ __global__ __launch_bounds__(256) void k_g2p1_tiled(GridGeom g, TileGeom tg, Par
         for (int i = beg + int(threadIdx.x); i < end; i += blockDim.x) {
             double x[3];
             load_x<DIM>(pp.x, i, x);
+            const R m0 = pp.ms[i];
+            R v0[3];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) v0[d] = pp.v[d][i];
             Stencil<DIM, S, R> st;
             int flag = 0;  // reported by the P2G #1 kernel
             if (!make_stencil<DIM, S, R>(g, x, st, flag)) {
@@ -586,7 +590,7 @@ __global__ __launch_bounds__(256) void k_g2p1_tiled(GridGeom g, TileGeom tg, Par
                 });
             }
             R pv[3] = {0, 0, 0};
-            g2p1_commit<DIM, R>(pp, sa, i, A, B, C, pv);
+            g2p1_commit<DIM, R>(pp, sa, i, A, B, C, pv, m0, v0);
 #pragma unroll
             for (int a = 0; a < DIM; ++a) pp.pm[a][i] = pv[a];
         }
