@@ -907,8 +907,8 @@ __device__ __forceinline__ int cell_of(const GridGeom& g, const double* x) {
 // The particle state G2P #2's commit reads.  Every load of the commit is
 // issued before any of its stores: the particle streams are separate arrays,
 // but the compiler cannot know they do not alias, so a load placed after a
-// store waits for it.  Callers load it before the gather, so the loads are in
-// flight while the stencil is summed.
+// store waits for it.  Where registers allow, callers load it before the
+// gather, so the loads are in flight while the stencil is summed.
 template <int DIM, typename R>
 struct CommitIn {
     static constexpr int NS = DIM == 2 ? 4 : 6;
@@ -917,6 +917,13 @@ struct CommitIn {
     R vol0, eq, ek, dxp[3];
     int nid;
 };
+
+// Whether a kernel loads the commit state before its gather: FP32 Drucker-
+// Prager / linear-elastic keep 3 (tiled) wavefronts per SIMD with the state
+// live across the gather (166 registers); the 3D Mohr-Coulomb and FP64 forms
+// would lose a wavefront per SIMD, so they load it after the gather.
+template <typename R, int CM>
+constexpr bool kCommitLoadsEarly = sizeof(R) == 4 && CM != 2;
 
 template <int DIM, typename R, int CM>
 __device__ __forceinline__ void load_commit_in(const PartPtrs<R>& pp, const StepArgs<R>& sa, int i,
@@ -1059,7 +1066,7 @@ __global__ __launch_bounds__(256) void k_g2p2(GridGeom g, PartPtrs<R> pp, GridPt
     double x[3];
     load_x<DIM>(pp.x, i, x);
     CommitIn<DIM, R> cin;
-    load_commit_in<DIM, R, CM>(pp, sa, i, cin);
+    if constexpr (kCommitLoadsEarly<R, CM>) load_commit_in<DIM, R, CM>(pp, sa, i, cin);
     Stencil<DIM, S, R> st;
     int flag = 0;
     if (!make_stencil<DIM, S, R>(g, x, st, flag)) {
@@ -1085,6 +1092,7 @@ __global__ __launch_bounds__(256) void k_g2p2(GridGeom g, PartPtrs<R> pp, GridPt
             }
         }
     });
+    if constexpr (!kCommitLoadsEarly<R, CM>) load_commit_in<DIM, R, CM>(pp, sa, i, cin);
     g2p2_commit<DIM, R, CM>(g, pp, gp, mats, sa, i, x, F, cin);
     raise_flag(flags, flag);
 }

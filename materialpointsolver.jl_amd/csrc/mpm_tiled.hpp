@@ -455,7 +455,7 @@ __global__ __launch_bounds__(256) void k_g2p2_tiled(
             double x[3];
             load_x<DIM>(pp.x, i, x);
             CommitIn<DIM, R> cin;
-            load_commit_in<DIM, R, CM>(pp, sa, i, cin);
+            if constexpr (kCommitLoadsEarly<R, CM>) load_commit_in<DIM, R, CM>(pp, sa, i, cin);
             Stencil<DIM, S, R> st;
             if (!make_stencil<DIM, S, R>(g, x, st, flag)) {
                 if (sa.mvl) pp.p2c[i] = -1;  // frozen particle: no cell
@@ -499,6 +499,7 @@ __global__ __launch_bounds__(256) void k_g2p2_tiled(
                     }
                 });
             }
+            if constexpr (!kCommitLoadsEarly<R, CM>) load_commit_in<DIM, R, CM>(pp, sa, i, cin);
             g2p2_commit<DIM, R, CM>(g, pp, gp, mats, sa, i, x, F, cin);
         }
     }
