@@ -136,6 +136,18 @@ __device__ __forceinline__ void stage_window(const GridGeom& g, const int* w0, R
     }
 }
 
+// FP32 3D P2G #1 windows interleave field pairs: (m, px), (py, pz), (fx, fy) as
+// 64-bit pairs per node and fz alone, so a stencil slot reads and writes three
+// pairs and one value (4 + 4 LDS instructions instead of 7 + 7).  Field f of
+// window node k sits at f < 6 ? (f / 2) 2 WN + 2 k + f % 2 : 6 WN + k; the
+// window size is unchanged.
+template <int DIM, int NF, typename R>
+constexpr bool kPairedP2G = DIM == 3 && NF == 7 && sizeof(R) == 4;
+
+__device__ __forceinline__ int paired_offset(int f, int k, int wn) {
+    return f < 6 ? (f >> 1) * 2 * wn + 2 * k + (f & 1) : 6 * wn + k;
+}
+
 // Flush a scatter window: global atomic adds of every non-zero node (only
 // window nodes inside the grid can be non-zero).
 template <int DIM, int WW, int ZS, int WN, int NF, typename R>
@@ -148,7 +160,7 @@ __device__ __forceinline__ void flush_window(const GridGeom& g, const int* w0, l
         bool any = false;
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-            vals[f] = win[f * WN + k];
+            vals[f] = win[kPairedP2G<DIM, NF, R> ? paired_offset(f, k, WN) : f * WN + k];
             any = any || vals[f] != R(0);
         }
         if (!any) continue;
@@ -227,6 +239,55 @@ __device__ __forceinline__ void p2g3_window(const Stencil<3, S, R>& st, R m, R v
     });
 }
 
+// p2g3_window on the paired FP32 layout (kPairedP2G): W is the window's first
+// float, node0 the stencil base's node index in the window.
+template <int S>
+__device__ __forceinline__ void p2g3_window_paired(const Stencil<3, S, float>& st, float m,
+                                                   float vol, const float* pv, const float* sg,
+                                                   const Gravity<float>& G, lds_ptr<float> W,
+                                                   int node0, int ws, int zs, int wn) {
+    using u64p = volatile __attribute__((address_space(3))) unsigned long long*;
+    auto lo = [](unsigned long long v) { return __uint_as_float(unsigned(v)); };
+    auto hi = [](unsigned long long v) { return __uint_as_float(unsigned(v >> 32)); };
+    auto pack = [](float a, float b) {
+        return (unsigned long long)__float_as_uint(a) |
+               ((unsigned long long)__float_as_uint(b) << 32);
+    };
+    const float mv = -vol;
+    const float sxx = mv * sg[0], sxy = mv * sg[3], sxz = mv * sg[5];
+    const float syy = mv * sg[1], syz = mv * sg[4], szz = mv * sg[2];
+    static_for<S>([&](auto c) {
+        static_for<S>([&](auto b) {
+            const float A = st.N[1][b] * st.N[2][c];
+            const float B = st.dN[1][b] * st.N[2][c];
+            const float C = st.N[1][b] * st.dN[2][c];
+            const float mA = m * A;
+            const float Px = sxx * A, Py = sxy * A, Pz = sxz * A;
+            const float Qx = sxy * B + sxz * C + mA * G.g[0];
+            const float Qy = syy * B + syz * C + mA * G.g[1];
+            const float Qz = syz * B + szz * C + mA * G.g[2];
+            static_for<S>([&](auto a) {
+                const float Nx = st.N[0][a], dNx = st.dN[0][a];
+                const float N = Nx * A;
+                if (N != 0.0f) {
+                    const int k = node0 + int(c) * zs + int(b) * ws + int(a);
+                    u64p P0 = reinterpret_cast<u64p>(W + 2 * k);
+                    u64p P1 = reinterpret_cast<u64p>(W + 2 * wn + 2 * k);
+                    u64p P2 = reinterpret_cast<u64p>(W + 4 * wn + 2 * k);
+                    lds_ptr<float> L6 = W + 6 * wn + k;
+                    const unsigned long long o0 = *P0, o1 = *P1, o2 = *P2;
+                    const float o6 = *L6;
+                    *P0 = pack(lo(o0) + m * N, hi(o0) + N * pv[0]);
+                    *P1 = pack(lo(o1) + N * pv[1], hi(o1) + N * pv[2]);
+                    *P2 = pack(lo(o2) + (dNx * Px + Nx * Qx), hi(o2) + (dNx * Py + Nx * Qy));
+                    *L6 = o6 + (dNx * Pz + Nx * Qz);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            });
+        });
+    });
+}
+
 // ---------------------------------------------------------------------------
 // Tile scatters.  MODE 0: P2G #1 (P2G_OS!, utils_OS.jl:424-496): mass,
 // momentum and internal + external force.  MODE 1: the momentum scatter of the
@@ -244,7 +305,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     constexpr int NF = MODE == 0 ? 1 + 2 * DIM : DIM;
     constexpr int NWIN = kScatterWindows;
     static_assert(NWIN == 1 || NWIN == 2, "one or two scatter windows");
-    __shared__ R win_[NWIN * NF * WN];
+    __shared__ __attribute__((aligned(16))) R win_[NWIN * NF * WN];
     __shared__ unsigned claim_[NWIN * W::NBN];
     lds_ptr<R> win = (lds_ptr<R>)win_;
     lds_ptr<unsigned> claim = (lds_ptr<unsigned>)claim_;
@@ -370,7 +431,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     tag += 64u;
                     auto lds_pass = [&] {
                         if constexpr (MODE == 0 && DIM == 3) {
-                            p2g3_window<S, R>(st, m, vol, pv, P.sg, G, win + wb, WW, ZS, WN);
+                            if constexpr (kPairedP2G<DIM, NF, R>)
+                                p2g3_window_paired<S>(st, m, vol, pv, P.sg, G, win + side * NF * WN,
+                                                      wb - side * NF * WN, WW, ZS, WN);
+                            else
+                                p2g3_window<S, R>(st, m, vol, pv, P.sg, G, win + wb, WW, ZS, WN);
                         } else {
                             for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R dNx, R dNy, R dNz) {
                                 if (N != R(0)) {
