@@ -141,11 +141,15 @@ __device__ __forceinline__ void stage_window(const GridGeom& g, const int* w0, R
 // pairs and one value (4 + 4 LDS instructions instead of 7 + 7).  Field f of
 // window node k sits at f < 6 ? (f / 2) 2 WN + 2 k + f % 2 : 6 WN + k; the
 // window size is unchanged.
+// The momentum scatter of the re-map (3 fields) pairs (px, py) the same way,
+// pz alone.
 template <int DIM, int NF, typename R>
-constexpr bool kPairedP2G = DIM == 3 && NF == 7 && sizeof(R) == 4;
+constexpr bool kPairedP2G = DIM == 3 && (NF == 7 || NF == 3) && sizeof(R) == 4;
 
+template <int NF>
 __device__ __forceinline__ int paired_offset(int f, int k, int wn) {
-    return f < 6 ? (f >> 1) * 2 * wn + 2 * k + (f & 1) : 6 * wn + k;
+    constexpr int NP = NF / 2;  // pairs; the last field is alone
+    return f < 2 * NP ? (f >> 1) * 2 * wn + 2 * k + (f & 1) : 2 * NP * wn + k;
 }
 
 // Flush a scatter window: global atomic adds of every non-zero node (only
@@ -160,7 +164,7 @@ __device__ __forceinline__ void flush_window(const GridGeom& g, const int* w0, l
         bool any = false;
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-            vals[f] = win[kPairedP2G<DIM, NF, R> ? paired_offset(f, k, WN) : f * WN + k];
+            vals[f] = win[kPairedP2G<DIM, NF, R> ? paired_offset<NF>(f, k, WN) : f * WN + k];
             any = any || vals[f] != R(0);
         }
         if (!any) continue;
@@ -436,6 +440,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                                       wb - side * NF * WN, WW, ZS, WN);
                             else
                                 p2g3_window<S, R>(st, m, vol, pv, P.sg, G, win + wb, WW, ZS, WN);
+                        } else if constexpr (MODE == 1 && kPairedP2G<DIM, NF, R>) {
+                            // (px, py) as one 64-bit read-add-write, pz alone
+                            using u64p = volatile __attribute__((address_space(3))) unsigned long long*;
+                            lds_ptr<R> Wb = win + side * NF * WN;
+                            const int node0 = wb - side * NF * WN;
+                            for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R, R, R) {
+                                if (N != R(0)) {
+                                    const int k = node0 + off;
+                                    u64p P0 = reinterpret_cast<u64p>(Wb + 2 * k);
+                                    lds_ptr<R> L2 = Wb + 2 * WN + k;
+                                    const unsigned long long o0 = *P0;
+                                    const R o2 = *L2;
+                                    const float a = __uint_as_float(unsigned(o0)) + pv[0] * N;
+                                    const float b = __uint_as_float(unsigned(o0 >> 32)) + pv[1] * N;
+                                    *P0 = (unsigned long long)__float_as_uint(a) |
+                                          ((unsigned long long)__float_as_uint(b) << 32);
+                                    *L2 = o2 + pv[2] * N;
+                                }
+                                __builtin_amdgcn_sched_barrier(0);
+                            });
                         } else {
                             for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R dNx, R dNy, R dNz) {
                                 if (N != R(0)) {
