@@ -143,8 +143,13 @@ __device__ __forceinline__ void stage_window(const GridGeom& g, const int* w0, R
 // window size is unchanged.
 // The momentum scatter of the re-map (3 fields) pairs (px, py) the same way,
 // pz alone.
-template <int DIM, int NF, typename R>
-constexpr bool kPairedP2G = DIM == 3 && (NF == 7 || NF == 3) && sizeof(R) == 4;
+// 64-bit LDS accesses need 8-byte alignment: every pair must start at an even
+// float, so the paired layout is used only when the field stride WN is even
+// (uGIMP windows: 6^3 and 10^2 nodes; the linear basis's 5^3 and 9^2 keep the
+// one-field-per-access layout).
+template <int DIM, int NF, typename R, int WN>
+constexpr bool kPairedP2G = sizeof(R) == 4 && WN % 2 == 0 &&
+                            ((DIM == 3 && (NF == 7 || NF == 3)) || (DIM == 2 && (NF == 5 || NF == 2)));
 
 template <int NF>
 __device__ __forceinline__ int paired_offset(int f, int k, int wn) {
@@ -164,7 +169,7 @@ __device__ __forceinline__ void flush_window(const GridGeom& g, const int* w0, l
         bool any = false;
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-            vals[f] = win[kPairedP2G<DIM, NF, R> ? paired_offset<NF>(f, k, WN) : f * WN + k];
+            vals[f] = win[kPairedP2G<DIM, NF, R, WN> ? paired_offset<NF>(f, k, WN) : f * WN + k];
             any = any || vals[f] != R(0);
         }
         if (!any) continue;
@@ -435,28 +440,37 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     tag += 64u;
                     auto lds_pass = [&] {
                         if constexpr (MODE == 0 && DIM == 3) {
-                            if constexpr (kPairedP2G<DIM, NF, R>)
+                            if constexpr (kPairedP2G<DIM, NF, R, WN>)
                                 p2g3_window_paired<S>(st, m, vol, pv, P.sg, G, win + side * NF * WN,
                                                       wb - side * NF * WN, WW, ZS, WN);
                             else
                                 p2g3_window<S, R>(st, m, vol, pv, P.sg, G, win + wb, WW, ZS, WN);
-                        } else if constexpr (MODE == 1 && kPairedP2G<DIM, NF, R>) {
-                            // (px, py) as one 64-bit read-add-write, pz alone
+                        } else if constexpr (kPairedP2G<DIM, NF, R, WN>) {
+                            // field pairs as 64-bit read-add-writes, an odd last field alone
                             using u64p = volatile __attribute__((address_space(3))) unsigned long long*;
+                            constexpr int NP = NF / 2;
                             lds_ptr<R> Wb = win + side * NF * WN;
                             const int node0 = wb - side * NF * WN;
-                            for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R, R, R) {
+                            for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R dNx, R dNy, R dNz) {
                                 if (N != R(0)) {
+                                    R c[NF];
+                                    contrib(N, dNx, dNy, dNz, c);
                                     const int k = node0 + off;
-                                    u64p P0 = reinterpret_cast<u64p>(Wb + 2 * k);
-                                    lds_ptr<R> L2 = Wb + 2 * WN + k;
-                                    const unsigned long long o0 = *P0;
-                                    const R o2 = *L2;
-                                    const float a = __uint_as_float(unsigned(o0)) + pv[0] * N;
-                                    const float b = __uint_as_float(unsigned(o0 >> 32)) + pv[1] * N;
-                                    *P0 = (unsigned long long)__float_as_uint(a) |
-                                          ((unsigned long long)__float_as_uint(b) << 32);
-                                    *L2 = o2 + pv[2] * N;
+                                    unsigned long long o[NP];
+#pragma unroll
+                                    for (int q = 0; q < NP; ++q)
+                                        o[q] = *reinterpret_cast<u64p>(Wb + 2 * q * WN + 2 * k);
+                                    R o1 = R(0);
+                                    if constexpr (NF % 2) o1 = Wb[2 * NP * WN + k];
+#pragma unroll
+                                    for (int q = 0; q < NP; ++q) {
+                                        const float a = __uint_as_float(unsigned(o[q])) + c[2 * q];
+                                        const float b = __uint_as_float(unsigned(o[q] >> 32)) + c[2 * q + 1];
+                                        *reinterpret_cast<u64p>(Wb + 2 * q * WN + 2 * k) =
+                                            (unsigned long long)__float_as_uint(a) |
+                                            ((unsigned long long)__float_as_uint(b) << 32);
+                                    }
+                                    if constexpr (NF % 2) Wb[2 * NP * WN + k] = o1 + c[NF - 1];
                                 }
                                 __builtin_amdgcn_sched_barrier(0);
                             });

@@ -233,15 +233,23 @@ def test_fp64_body_force_extension(mpm, orc):
     _compare(mpm, orc, sc2, 100, 1e-9)
 
 
-def test_fp32_tiled_kernels_match_plain_kernels(mpm):
+@pytest.mark.parametrize("dim,basis", [(3, "uGIMP"), (3, "linear"), (2, "uGIMP"), (2, "linear")])
+def test_fp32_tiled_kernels_match_plain_kernels(mpm, dim, basis):
     # A dense FP32 column right after the sort, where most rounds of the tile
     # scatters are full (64 particles on the 64 bases of their tile): the LDS
     # tile kernels and the plain global-atomic kernels sum the same terms, so the
     # two differ by FP32 summation order only (a missing or misplaced term would
     # show at 1e-2 and above).  Stresses come from differences of nearly equal
     # nodal velocities, which amplifies that rounding; the bounds leave room.
-    args, grid, mp, attr, bc = mpm.scenes.column_collapse_3d(
-        h=0.01, width=0.16, height=0.08, box_xy=0.4, box_z=0.12, eps="FP32", nsteps=10)
+    # (FP32 uGIMP windows interleave field pairs; linear-basis windows do not)
+    if dim == 3:
+        args, grid, mp, attr, bc = mpm.scenes.column_collapse_3d(
+            h=0.01, width=0.16, height=0.08, box_xy=0.4, box_z=0.12, eps="FP32", nsteps=10,
+            basis=basis)
+    else:
+        args, grid, mp, attr, bc = mpm.scenes.column_collapse_2d(
+            h=0.005, width=0.2, height=0.1, box_x=0.5, box_y=0.14, eps="FP32", nsteps=10,
+            basis=basis)
     dt = float(args.ΔT)
     x0 = np.asarray(mp.ξ, dtype=np.float64)
     tiled = mpm.HIPModel(grid, mp, attr, bc, sort_interval=10)
