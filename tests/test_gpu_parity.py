@@ -240,7 +240,7 @@ def test_fp32_tiled_kernels_match_plain_kernels(mpm, dim, basis):
     # tile kernels and the plain global-atomic kernels sum the same terms, so the
     # two differ by FP32 summation order only (a missing or misplaced term would
     # show at 1e-2 and above).  Stresses come from differences of nearly equal
-    # nodal velocities, which amplifies that rounding; the bounds leave room.
+    # nodal velocities, which amplifies that rounding; the bounds leave 10x room.
     # (FP32 uGIMP windows interleave field pairs; linear-basis windows do not)
     if dim == 3:
         args, grid, mp, attr, bc = mpm.scenes.column_collapse_3d(
@@ -258,7 +258,8 @@ def test_fp32_tiled_kernels_match_plain_kernels(mpm, dim, basis):
         plain.set_option("tiles", False)
         sa = tiled.step_args(args)
         T = 0.0
-        for nsteps, tol in ((1, 1e-4), (4, 1e-3)):
+        # measured: rel L2 ~1e-7 (x, v) and ~7e-7 (σ) after one step, on every case
+        for nsteps, tol in ((1, 1e-5), (4, 1e-4)):
             tiled.run(sa, dt, T, nsteps)
             T = plain.run(sa, dt, T, nsteps)
             a, b = tiled.download(), plain.download()
