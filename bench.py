@@ -8,10 +8,11 @@ particles cell-sorted every 10 steps.  A step is one full MUSL timestep of
 every particle (both P2G passes, both grid solves, G2P, Drucker-Prager).
 
 Prints one JSON line: particle-steps/s over the timed steps, the roofline of
-the dominant kernel (algorithmic bytes per launch / its mean HIP-event
-duration on the context stream, against the 8 TB/s HBM peak) and the CPU
-baseline (the FP64 oracle restatement of the reference's CPU path, OpenMP on
-the host cores, on a bounded sample of the same workload).
+P2G #1 (algorithmic bytes per launch / its mean HIP-event duration on the
+context stream inside the timed region, against the 8 TB/s HBM peak) with
+G2P #2 beside it, and the CPU baseline (the FP64 oracle restatement of the
+reference's CPU path, OpenMP on every core this process may use and on one
+thread, on a bounded sample of the same workload).
 """
 from __future__ import annotations
 
@@ -38,6 +39,23 @@ ALGO_BYTES = {
     "k_grid2": (0, 29),
     "k_g2p2": (208, 12),
 }
+# Kernels outside the step's compulsory traffic (the sort is an implementation
+# choice and counts against the step's fraction): their own minimum bytes, for
+# the per-kernel table only.  k_sort: positions in, key and rank out, then key
+# and rank in, order and sorted key out (48 B); k_permute: every permuted FP32
+# stream read and written once (124 B each way) plus the order (4 B); MVL halves
+# and the learned return: the particle fields each reads and writes.
+AUX_BYTES = {
+    "k_sort": (48, 0),
+    "k_permute": (252, 0),
+    "k_mvl1": (56, 0),
+    "k_mvl2": (48, 0),
+    "k_learned_return": (88, 0),
+}
+# The kernel the roofline figure is quoted for: P2G #1, the per-kernel gate of
+# the target (P2G and G2P at >= 50 % of HBM peak); G2P #2 is reported beside it.
+ROOFLINE_KERNEL = "k_p2g"
+SECOND_KERNEL = "k_g2p2"
 
 
 def parse():
@@ -50,7 +68,10 @@ def parse():
     ap.add_argument("--h", type=float, default=0.005, help="grid spacing (0.005 = 2,048,000 particles)")
     ap.add_argument("--cpu-steps", type=int, default=40,
                     help="oracle steps for the CPU baseline (about 12 s on 16 cores)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(cores, 16)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0 = every core this process may run on (sched_getaffinity)")
+    ap.add_argument("--cpu-steps-1t", type=int, default=2,
+                    help="oracle steps of the single-thread CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs (FETCH_SIZE, WRITE_SIZE)")
@@ -227,30 +248,24 @@ def main():
           file=sys.stderr, flush=True)
 
     n = int(mp.np)
-    # warmup; its last steps carry events on every kernel to find the dominant one
-    pre = min(a.warmup, 20)
-    Ti = model.run(sargs, dt, 0.0, a.warmup - pre)
-    model.profile_enable(True)
-    Ti = model.run(sargs, dt, Ti, pre)
-    pre_prof = model.profile_read()
-    model.profile_enable(False)
-    dom = max(pre_prof, key=lambda k: pre_prof[k][0]) if pre else "k_p2g"
+    Ti = model.run(sargs, dt, 0.0, a.warmup)
     model.sync()
     n_active = model.count_active_nodes()
+    dom = ROOFLINE_KERNEL
 
     def table(prof):
         kern = {}
         for name, (ms, cnt) in prof.items():
             if cnt == 0:
                 continue
-            per_p, per_n = ALGO_BYTES.get(name, (0, 0))
+            per_p, per_n = ALGO_BYTES.get(name) or AUX_BYTES.get(name, (0, 0))
             algo = per_p * n + per_n * n_active
             avg_s = ms * 1e-3 / cnt
             kern[name] = dict(avg_us=avg_s * 1e6, launches=cnt, total_ms=ms,
                               algo_bytes=algo, gbs=(algo / avg_s / 1e9) if algo else None)
         return kern
 
-    # timed region: HIP events around the dominant kernel's launches only
+    # timed region: HIP events (on the context stream) around P2G #1's launches only
     model.set_option("profile_kernels", [dom])
     model.profile_enable(True)
     model.sync()
@@ -277,7 +292,7 @@ def main():
     ms_all_events = (time.perf_counter() - t0) / a.steps * 1e3
     kernels = table(model.profile_read())
     model.profile_enable(False)
-    print(f"# dominant {dom}: {dk['avg_us']:.1f} us in the timed region; per-kernel table of "
+    print(f"# roofline kernel {dom}: {dk['avg_us']:.1f} us in the timed region; per-kernel table of "
           f"the next {a.steps} steps ({ms_all_events:.4f} ms/step with events on every kernel):",
           file=sys.stderr, flush=True)
     for name, k in sorted(kernels.items(), key=lambda kv: -kv[1]["total_ms"]):
@@ -288,9 +303,9 @@ def main():
     step_bytes = sum(ALGO_BYTES[k][0] * n + ALGO_BYTES[k][1] * n_active for k in ALGO_BYTES)
     tuned = None
     if not a.no_autotune:
-        best, table, Ti = mpm.autotune(model, sargs, dt, Ti, steps=40, warm=10)
+        best, tab, Ti = mpm.autotune(model, sargs, dt, Ti, steps=40, warm=10)
         tuned = {"best": best, "ms_per_step": {f"sort{k}_{'tiles' if t else 'plain'}": round(s * 1e3, 4)
-                                                for k, t, s in table}}
+                                                for k, t, s in tab}}
         print(f"# autotune: {best}", file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu_baseline:
@@ -301,10 +316,14 @@ def main():
         traffic, traffic_note = measure_traffic(a)
         if traffic is None:
             print(f"# traffic unavailable: {traffic_note}", file=sys.stderr, flush=True)
-    dom_traffic = None
+    dom_traffic = sec_traffic = None
     if traffic and dom in traffic["kernels"]:
         t = traffic["kernels"][dom]
         dom_traffic = t["fetch_bytes"] + t["write_bytes"]
+    if traffic and SECOND_KERNEL in traffic["kernels"]:
+        t = traffic["kernels"][SECOND_KERNEL]
+        sec_traffic = t["fetch_bytes"] + t["write_bytes"]
+    if traffic:
         for k, t in traffic["kernels"].items():
             print(f"#   traffic {k:14s} fetch {t['fetch_bytes'] / 1e6:8.1f} MB  write "
                   f"{t['write_bytes'] / 1e6:8.1f} MB  (algorithmic "
@@ -351,7 +370,10 @@ def main():
             "algorithmic_bytes": dk["algo_bytes"],
             "avg_launch_us": dk["avg_us"],
             "launches": dk["launches"],
+            "timing": "HIP events on the context stream around every launch of this kernel "
+                      "inside the timed region (no other kernel carries events there)",
         },
+        "roofline_g2p2": _second_roofline(kernels.get(SECOND_KERNEL), sec_traffic),
         "cpu_baseline": cpu,
         "autotune": tuned,
     }
@@ -368,6 +390,17 @@ def main():
     if not a.no_slope:
         out["slope"] = slope_leg(a, mpm)
     print(json.dumps(out), flush=True)
+
+
+def _second_roofline(k, traffic):
+    """G2P #2 (the other kernel of the per-kernel gate) from the per-kernel table."""
+    if not k or not k["gbs"]:
+        return None
+    return {"kernel": SECOND_KERNEL, "bound": "hbm", "achieved": k["gbs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": k["gbs"] / HBM_PEAK_GBS, "traffic": traffic,
+            "algorithmic_bytes": k["algo_bytes"], "avg_launch_us": k["avg_us"],
+            "launches": k["launches"],
+            "timing": "HIP events around every kernel of the steps after the timed region"}
 
 
 def ensemble_leg(a, mpm):
@@ -670,22 +703,37 @@ def slope_leg(a, mpm):
 
 
 def cpu_baseline(a, args, grid, mp, attr, bc):
-    """The FP64 oracle (CPU restatement of the reference step) on the same scene."""
+    """The FP64 oracle (CPU restatement of the reference step) on the same scene,
+    on every core this process may use and on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
-    o = oracle.Oracle(args, grid, mp, attr, bc, nthreads=threads)
-    dt = float(args.ΔT)
-    o.run(dt, 0.0, 1)  # first touch of every array
-    t0 = time.perf_counter()
-    o.run(dt, dt, a.cpu_steps)
-    el = time.perf_counter() - t0
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    threads = a.cpu_threads or allowed
     n = int(mp.np)
-    return {"value": n * a.cpu_steps / el, "unit": "particle-steps/s", "cores": threads,
+    dt = float(args.ΔT)
+    res = {}
+    for nt, steps in ((threads, a.cpu_steps), (1, a.cpu_steps_1t)):
+        o = oracle.Oracle(args, grid, mp, attr, bc, nthreads=nt)
+        o.run(dt, 0.0, 1)  # first touch of every array
+        t0 = time.perf_counter()
+        o.run(dt, dt, steps)
+        res[nt] = (steps, time.perf_counter() - t0)
+        del o
+        print(f"# cpu baseline: {nt} thread(s), {steps} steps in {res[nt][1]:.1f} s",
+              file=sys.stderr, flush=True)
+    steps, el = res[threads]
+    s1, e1 = res[1]
+    return {"value": n * steps / el, "unit": "particle-steps/s", "cores": threads,
             "kind": "port",
-            "sample": f"same 3D column ({n} particles), {a.cpu_steps} FP64 steps after one "
-                      f"untimed step, OpenMP {threads} threads, {el:.1f} s"}
+            "sample": f"same 3D column ({n} particles), {steps} FP64 steps after one untimed "
+                      f"step, OpenMP {threads} threads (every core this process may run on), "
+                      f"{el:.1f} s",
+            "single_thread": {"value": n * s1 / e1, "cores": 1,
+                              "sample": f"{s1} FP64 steps after one untimed step, {e1:.1f} s"}}
 
 
 if __name__ == "__main__":
