@@ -303,9 +303,10 @@ def main():
     step_bytes = sum(ALGO_BYTES[k][0] * n + ALGO_BYTES[k][1] * n_active for k in ALGO_BYTES)
     tuned = None
     if not a.no_autotune:
-        best, tab, Ti = mpm.autotune(model, sargs, dt, Ti, steps=40, warm=10)
-        tuned = {"best": best, "ms_per_step": {f"sort{k}_{'tiles' if t else 'plain'}": round(s * 1e3, 4)
-                                                for k, t, s in tab}}
+        best, tab, Ti = mpm.autotune(model, sargs, dt, Ti, steps=40, warm=10,
+                                     scatters=mpm.SCATTER_FORMS)
+        tuned = {"best": best, "ms_per_step": {f"sort{k}_{sc if t else 'plain'}": round(s * 1e3, 4)
+                                                for k, t, sc, s in tab}}
         print(f"# autotune: {best}", file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu_baseline:

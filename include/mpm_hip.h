@@ -118,12 +118,18 @@ int mpm_upload_particles(mpm_ctx *ctx, const double *xi, const double *vs,
                          const double *rho0, const double *eps_q, const double *eps_k,
                          const int64_t *nid);
 
-/* Upload the material table (nmat entries each; attr.Ks, Gs, ϕ, ψ, c, σt, Hp, cr;
- * src/types/property.jl:31-52).  Replaces host2device for attr
- * (devicehelpfunc_amd.jl:27). */
+/* Upload the material table (nmat entries each; attr.Ks, Gs, ϕ, ψ, c, σt, Hp, cr,
+ * ϕr; src/types/property.jl:31-52).  Replaces host2device for attr
+ * (devicehelpfunc_amd.jl:27).  ϕr (may be NULL = all zero) is read by the 3D
+ * Mohr-Coulomb return only: with Hp < 0, c > cr and ϕr > 0, friction softens
+ * linearly from ϕ to ϕr and dilation from ψ to 0 over the plastic strain in
+ * which cohesion softens from c to cr (mohr_coulomb.tcc:8-25); the reference's
+ * 2D mcP! softens cohesion only (mohrcoulomb.jl:36), and so does this library
+ * in 2D. */
 int mpm_upload_materials(mpm_ctx *ctx, const double *Ks, const double *Gs,
                          const double *phi, const double *psi, const double *c,
-                         const double *sigma_t, const double *Hp, const double *cr);
+                         const double *sigma_t, const double *Hp, const double *cr,
+                         const double *phir);
 
 /* Upload the solid velocity boundary (bc.v{x,y,z}_s_idx / _val, ni entries each in
  * the reference's node order; vz_* ignored in 2D; src/types/boundary.jl:115-131,169-190).
@@ -161,8 +167,17 @@ int mpm_run_graph(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int6
  *  MPM_OPT_PROFILE_MASK:  bit k set = mpm_profile_enable times kernel k of
  *                         mpm_profile_kernel_name's order; 0 = every kernel
  *                         (default).  Fewer event pairs per step keep the
- *                         profiled steps closer to unprofiled ones. */
-enum { MPM_OPT_SORT_INTERVAL = 0, MPM_OPT_TILES = 1, MPM_OPT_PROFILE_MASK = 2 };
+ *                         profiled steps closer to unprofiled ones.
+ *  MPM_OPT_SCATTER:       form of the tile scatters (P2G and the MUSL re-map):
+ *                         MPM_SCATTER_OWNED (default): each lane owns one
+ *                         stencil base of a tile and sums its particles in
+ *                         registers (3D P2G: mass and momentum, then force);
+ *                         MPM_SCATTER_OWNED_LAYERS: the same, one z layer of
+ *                         the stencil per pass (fewer registers, more waves);
+ *                         MPM_SCATTER_ROUNDS: rank rounds of distinct bases
+ *                         adding into an LDS window. */
+enum { MPM_OPT_SORT_INTERVAL = 0, MPM_OPT_TILES = 1, MPM_OPT_PROFILE_MASK = 2, MPM_OPT_SCATTER = 3 };
+enum { MPM_SCATTER_ROUNDS = 0, MPM_SCATTER_OWNED = 1, MPM_SCATTER_OWNED_LAYERS = 2 };
 int mpm_set_option(mpm_ctx *ctx, int option, int64_t value);
 
 /* Wait for all work on the context's stream; reports range/finite flags the

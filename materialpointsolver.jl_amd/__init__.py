@@ -18,10 +18,10 @@ from .solver import (HIPModel, KAsync, Tpeak, clean_device, device2host, getBack
                      host2device, materialpointsolver, perf, procedure, submit_work)
 from . import scenes
 from .surrogate import Surrogate
-from .autotune import autotune
+from .autotune import SCATTER_FORMS, autotune
 
 __all__ = ["ArgumentError", "UserArgs2D", "UserArgs3D", "UserGrid2D", "UserGrid3D",
            "UserParticle2D", "UserParticle3D", "UserProperty", "UserVBoundary2D",
            "UserVBoundary3D", "julia_range", "meshbuilder", "HIPModel", "KAsync", "Tpeak",
            "clean_device", "device2host", "getBackend", "host2device", "materialpointsolver",
-           "perf", "procedure", "submit_work", "scenes", "Surrogate", "autotune"]
+           "perf", "procedure", "submit_work", "scenes", "Surrogate", "autotune", "SCATTER_FORMS"]

@@ -22,7 +22,10 @@ MPM_UGIMP, MPM_LINEAR = 0, 1
 MPM_MUSL, MPM_USL, MPM_USF = 0, 1, 2
 SCHEME = {"MUSL": MPM_MUSL, "USL": MPM_USL, "USF": MPM_USF}
 MPM_VA_A, MPM_VA_V = 0, 1
-MPM_OPT_SORT_INTERVAL, MPM_OPT_TILES, MPM_OPT_PROFILE_MASK = 0, 1, 2
+MPM_OPT_SORT_INTERVAL, MPM_OPT_TILES, MPM_OPT_PROFILE_MASK, MPM_OPT_SCATTER = 0, 1, 2, 3
+MPM_SCATTER_ROUNDS, MPM_SCATTER_OWNED, MPM_SCATTER_OWNED_LAYERS = 0, 1, 2
+SCATTER = {"rounds": MPM_SCATTER_ROUNDS, "owned": MPM_SCATTER_OWNED,
+           "owned_layers": MPM_SCATTER_OWNED_LAYERS}
 CONSTITUTIVE = {"linearelastic": 0, "druckerprager": 1, "mohrcoulomb": 2}
 
 
@@ -61,7 +64,7 @@ _PROTOS = {
     "mpm_abi_version": (C.c_int, []),
     "mpm_create": (C.c_int, [C.POINTER(MPMConfig), C.POINTER(_CTX)]),
     "mpm_upload_particles": (C.c_int, [_CTX, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _I64]),
-    "mpm_upload_materials": (C.c_int, [_CTX, _D, _D, _D, _D, _D, _D, _D, _D]),
+    "mpm_upload_materials": (C.c_int, [_CTX, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
     "mpm_upload_bc": (C.c_int, [_CTX, _I64, _D, _I64, _D, _I64, _D]),
     "mpm_step": (C.c_int, [_CTX, C.POINTER(MPMArgs), C.c_double, C.c_double]),
     "mpm_run": (C.c_int, [_CTX, C.POINTER(MPMArgs), C.c_double, C.c_double, C.c_int64, _D]),

@@ -1,6 +1,17 @@
-"""Autotuner for the step's runtime knobs: the sort interval and whether the
-LDS tile kernels are used (the tile shape and window size are compile-time:
-4x4x4 bases in 3D, 8x8 in 2D, windows without slack, two per scatter wavefront).
+"""Autotuner for the step's runtime knobs: the sort interval, whether the LDS
+tile kernels are used, and which tile-scatter form runs (P2G and the MUSL
+re-map):
+
+* ``"owned"``: each lane owns one stencil base of a tile and sums its dual
+  cell's particles in registers (3D P2G: mass and momentum, then force; two
+  passes, ~190 VGPRs, two wavefronts per SIMD);
+* ``"owned_layers"``: the same with one z layer of the stencil per pass (six
+  passes, ~130 VGPRs, three wavefronts per SIMD; more particle reloads);
+* ``"rounds"``: rank rounds of distinct bases adding into two LDS windows per
+  wavefront (12.6 KB of LDS per wavefront).
+
+The tile shape (4x4x4 bases in 3D, 8x8 in 2D) and the windows (no slack) are
+compile-time.
 
 ``autotune`` times candidate settings on the live model, in place: each
 candidate runs ``warm`` untimed steps and ``steps`` timed steps of the actual
@@ -11,23 +22,32 @@ from __future__ import annotations
 
 import time
 
+SCATTER_FORMS = ("owned", "owned_layers", "rounds")
+
 
 def autotune(model, sargs, dt, Ti, *, sort_intervals=(5, 10, 20, 40), tiles=(True, False),
-             steps=40, warm=10, graph=False, clock=time.perf_counter):
-    """Returns (best, table, Ti): best = dict(sort_interval=..., tiles=...), table =
-    list of (sort_interval, tiles, seconds per step), Ti = time after the trials."""
+             scatters=("owned",), steps=40, warm=10, graph=False, clock=time.perf_counter):
+    """Returns (best, table, Ti): best = dict(sort_interval=..., tiles=...,
+    scatter=...), table = list of (sort_interval, tiles, scatter, seconds per
+    step), Ti = time after the trials.  The scatter form only matters with
+    tiles; without tiles it is recorded as None."""
     table = []
     for t in tiles:
-        for k in sort_intervals:
-            model.set_option("tiles", t)
-            model.set_option("sort_interval", k)
-            Ti = model.run(sargs, dt, Ti, warm, graph=graph)
-            model.sync()
-            t0 = clock()
-            Ti = model.run(sargs, dt, Ti, steps, graph=graph)
-            model.sync()
-            table.append((k, bool(t), (clock() - t0) / steps))
-    k, t, _ = min(table, key=lambda r: r[2])
+        for sc in (scatters if t else (None,)):
+            for k in sort_intervals:
+                model.set_option("tiles", t)
+                if sc is not None:
+                    model.set_option("scatter", sc)
+                model.set_option("sort_interval", k)
+                Ti = model.run(sargs, dt, Ti, warm, graph=graph)
+                model.sync()
+                t0 = clock()
+                Ti = model.run(sargs, dt, Ti, steps, graph=graph)
+                model.sync()
+                table.append((k, bool(t), sc, (clock() - t0) / steps))
+    k, t, sc, _ = min(table, key=lambda r: r[3])
     model.set_option("tiles", t)
+    if sc is not None:
+        model.set_option("scatter", sc)
     model.set_option("sort_interval", k)
-    return dict(sort_interval=k, tiles=t), table, Ti
+    return dict(sort_interval=k, tiles=t, scatter=sc), table, Ti

@@ -14,6 +14,7 @@
 
 #include "../../include/mpm_hip.h"
 #include "mpm_kernels.hpp"
+#include "mpm_owned.hpp"
 #include "mpm_sort.hpp"
 #include "mpm_surrogate.hpp"
 
@@ -66,6 +67,7 @@ struct mpm_ctx {
     bool prof_on = false;
     uint64_t prof_mask = 0;  // kernel ids to time; 0 = all
     bool use_tiles = true;  // LDS-binned scatters when the particles are tile-sorted
+    int scatter = MPM_SCATTER_OWNED;  // tile scatter form (MPM_OPT_SCATTER)
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> pool;
     // captured step graphs of mpm_run_graph, keyed by what their kernel
@@ -132,7 +134,7 @@ struct ImplBase {
                                  const int64_t*) = 0;
     virtual int upload_materials(mpm_ctx*, const double*, const double*, const double*,
                                  const double*, const double*, const double*, const double*,
-                                 const double*) = 0;
+                                 const double*, const double*) = 0;
     virtual int upload_bc(mpm_ctx*, const int64_t*, const double*, const int64_t*, const double*,
                           const int64_t*, const double*) = 0;
     virtual int step(mpm_ctx*, const mpm_args& a, double dt, double Ti) = 0;
@@ -406,7 +408,7 @@ struct Impl : ImplBase {
 
     int upload_materials(mpm_ctx* c, const double* Ks, const double* Gs, const double* phi,
                          const double* psi, const double* coh, const double* sigt,
-                         const double* Hp, const double* cr) override {
+                         const double* Hp, const double* cr, const double* phir) override {
         if (!Ks || !Gs || !phi || !psi || !coh || !sigt || !Hp || !cr)
             return c->fail(MPM_E_ARG, "mpm_upload_materials: NULL array");
         const long long nm = c->cfg.nmat;
@@ -440,6 +442,11 @@ struct Impl : ImplBase {
             m.c = R(coh[k]);
             m.Hp = R(Hp[k]);
             m.cr = R(cr[k]);
+            // 3D Mohr-Coulomb softening of friction and dilation (mc_strength):
+            // over the plastic strain in which cohesion softens from c to cr
+            const double pr = phir ? phir[k] : 0.0;
+            m.phir = R(pr);
+            m.eqr = (Hp[k] < 0.0 && coh[k] > cr[k] && pr > 0.0) ? R((cr[k] - coh[k]) / Hp[k]) : R(0);
         }
         hipError_t e = hipMemcpy(d_mats, M.data(), sizeof(Material<R>) * nm, hipMemcpyHostToDevice);
         return e == hipSuccess ? MPM_OK : c->hip_fail(e, "upload materials");
@@ -523,10 +530,32 @@ struct Impl : ImplBase {
         int rc;
         (void)cf;
         const bool tiled = sort.enabled && sort.tiles_valid && c->use_tiles;
+        const bool owned = c->scatter != MPM_SCATTER_ROUNDS;
         constexpr int TSH = DIM == 3 ? 2 : 3;
+        // Base-owned scatter passes (OwnedPasses in mpm_owned.hpp): FP64 3D
+        // always takes one z layer per pass (its FP64 sums would not fit the
+        // register file); FP32 3D P2G #1 sums mass and momentum, then force
+        // (MPM_SCATTER_OWNED) or also one z layer per pass
+        // (MPM_SCATTER_OWNED_LAYERS); 2D sums everything in one pass.
+        const bool layers = DIM == 3 && (sizeof(R) == 8 || c->scatter == MPM_SCATTER_OWNED_LAYERS);
+        auto owned_scatter = [&](auto mode) {
+            constexpr int MODE = decltype(mode)::value;
+            constexpr int SP_FIELDS = MODE == 0 ? 1 : 0;
+            constexpr bool F64 = sizeof(R) == 8;
+            if (DIM == 3 && (F64 || layers))
+                k_scatter_owned<DIM, S, TSH, R, MODE, DIM == 3 ? 2 : 0><<<sort.tiled_grid(), 64, 0, s>>>(
+                    geom, sort.tg, pp, gp, Gv, sort.masks, sort.offs, sort.ovf, sort.rounds,
+                    sort.tile_list, sort.seg_begin, sort.seg_end, sort.tile_count, fl);
+            else if constexpr (!F64 || DIM == 2)
+                k_scatter_owned<DIM, S, TSH, R, MODE, DIM == 3 ? SP_FIELDS : 0><<<sort.tiled_grid(), 64, 0, s>>>(
+                    geom, sort.tg, pp, gp, Gv, sort.masks, sort.offs, sort.ovf, sort.rounds,
+                    sort.tile_list, sort.seg_begin, sort.seg_end, sort.tile_count, fl);
+        };
         if (phase != PH_HEAD) {
             if ((rc = c->launch(KID_P2G, [&] {
-                     if (tiled)
+                     if (tiled && owned)
+                         owned_scatter(std::integral_constant<int, 0>{});
+                     else if (tiled)
                          k_scatter_tiled<DIM, S, TSH, R, 0><<<sort.tiled_grid(), 64, 0, s>>>(
                              geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
                              sort.seg_begin, sort.seg_end, sort.tile_count, fl);
@@ -556,9 +585,12 @@ struct Impl : ImplBase {
                 return rc;
             if (tiled && sa.remap) {
                 if ((rc = c->launch(KID_P2G2, [&] {
-                         k_scatter_tiled<DIM, S, TSH, R, 1><<<sort.tiled_grid(), 64, 0, s>>>(
-                             geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
-                             sort.seg_begin, sort.seg_end, sort.tile_count, fl);
+                         if (owned)
+                             owned_scatter(std::integral_constant<int, 1>{});
+                         else
+                             k_scatter_tiled<DIM, S, TSH, R, 1><<<sort.tiled_grid(), 64, 0, s>>>(
+                                 geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
+                                 sort.seg_begin, sort.seg_end, sort.tile_count, fl);
                      })) != MPM_OK)
                     return rc;
             }
@@ -971,9 +1003,9 @@ int mpm_upload_particles(mpm_ctx* c, const double* xi, const double* vs, const d
 
 int mpm_upload_materials(mpm_ctx* c, const double* Ks, const double* Gs, const double* phi,
                          const double* psi, const double* coh, const double* sigma_t,
-                         const double* Hp, const double* cr) {
+                         const double* Hp, const double* cr, const double* phir) {
     if (!c) return MPM_E_ARG;
-    int rc = c->impl->upload_materials(c, Ks, Gs, phi, psi, coh, sigma_t, Hp, cr);
+    int rc = c->impl->upload_materials(c, Ks, Gs, phi, psi, coh, sigma_t, Hp, cr, phir);
     if (rc == MPM_OK) c->have_materials = true;
     return rc;
 }
@@ -1124,6 +1156,11 @@ int mpm_set_option(mpm_ctx* c, int option, int64_t value) {
         c->cfg.sort_interval = int32_t(value);
     } else if (option == MPM_OPT_TILES) {
         c->use_tiles = value != 0;
+    } else if (option == MPM_OPT_SCATTER) {
+        if (value != MPM_SCATTER_ROUNDS && value != MPM_SCATTER_OWNED && value != MPM_SCATTER_OWNED_LAYERS)
+            return c->fail(MPM_E_ARG, "mpm_set_option: scatter form is MPM_SCATTER_ROUNDS, "
+                                      "MPM_SCATTER_OWNED or MPM_SCATTER_OWNED_LAYERS");
+        c->scatter = int(value);
     } else if (option == MPM_OPT_PROFILE_MASK) {
         if (value < 0 || (kNumKernels < 63 && value >= (int64_t(1) << kNumKernels)))
             return c->fail(MPM_E_ARG, "mpm_set_option: profile mask names no such kernel");

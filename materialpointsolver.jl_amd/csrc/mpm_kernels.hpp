@@ -116,6 +116,8 @@ struct Material {
     R sinphi, cosphi, tanphi, c, Hp, cr; // Mohr-Coulomb
     R sinpsi;                            // Mohr-Coulomb (3D)
     R phi, psi;                          // angles (rad), inputs of the learned return map
+    R phir;                              // residual friction (3D Mohr-Coulomb softening)
+    R eqr;                               // residual plastic strain; 0: phi, psi do not soften
 };
 
 // Body force per unit mass on each axis.  The reference applies gravity on
@@ -827,6 +829,37 @@ __device__ __forceinline__ R mc_cohesion(const Material<R>& M, R eq) {
     return (c > M.cr || c != c) ? c : M.cr;  // max(c + Hp ϵq, cr)
 }
 
+// Strength in effect at plastic strain eq (3D): the linear softening rule of
+// mohr_coulomb.tcc:8-25 with one peak/residual strain pair for cohesion,
+// friction and dilation, peak at eq = 0.  Cohesion follows max(c + Hp ϵq, cr);
+// when Hp < 0, c > cr and ϕr > 0 (eqr = (cr - c) / Hp > 0, set at upload)
+// friction goes linearly from ϕ to ϕr and dilation from ψ to 0 over [0, eqr].
+template <typename R>
+struct MCStrength {
+    R c, phi, psi, sinphi, cosphi, sinpsi;
+};
+
+template <typename R>
+__device__ __forceinline__ MCStrength<R> mc_strength(const Material<R>& M, R eq) {
+    MCStrength<R> p;
+    p.c = mc_cohesion<R>(M, eq);
+    if (M.eqr > R(0)) {
+        const R t = eq >= M.eqr ? R(1) : eq / M.eqr;
+        p.phi = M.phi + (M.phir - M.phi) * t;
+        p.psi = M.psi - M.psi * t;
+        p.sinphi = sin(p.phi);
+        p.cosphi = cos(p.phi);
+        p.sinpsi = sin(p.psi);
+    } else {
+        p.phi = M.phi;
+        p.psi = M.psi;
+        p.sinphi = M.sinphi;
+        p.cosphi = M.cosphi;
+        p.sinpsi = M.sinpsi;
+    }
+    return p;
+}
+
 // Plastic strain bookkeeping of a stress correction d = s_new - s_trial
 // (inverse elastic matrix; ϵk assigned, ϵq accumulated).
 template <typename R>
@@ -845,10 +878,10 @@ __device__ __forceinline__ void mc_strain(const R* d, R& eq, R& ek, const Materi
 // Returns the kind of return (0 elastic, 1 main plane, 2/3 edge, 4 apex).
 template <typename R>
 __device__ __forceinline__ int mcP3(R* s, R& eq, R& ek, const Material<R>& M) {
-    const R c = mc_cohesion<R>(M, eq);
+    const MCStrength<R> P = mc_strength<R>(M, eq);
     R lam[3], V[3][3];
     sym3_eig<R>(s, lam, V);
-    const int kind = mc3_principal<R>(lam, M.Dt, M.Dd, M.sinphi, M.cosphi, M.sinpsi, c);
+    const int kind = mc3_principal<R>(lam, M.Dt, M.Dd, P.sinphi, P.cosphi, P.sinpsi, P.c);
     if (kind == 0) {
         ek = R(0);
         return 0;
@@ -881,10 +914,11 @@ __device__ __forceinline__ void log_return(const StepArgs<R>& sa, const R* s0, c
         x[6 + k] = float(e[k]);
         y[k] = float(s1[k] - s0[k]);
     }
+    const MCStrength<R> P = mc_strength<R>(M, eq0);
     x[12] = float(eq0);
-    x[13] = float(M.phi);
-    x[14] = float(mc_cohesion<R>(M, eq0));
-    x[15] = float(M.psi);
+    x[13] = float(P.phi);
+    x[14] = float(P.c);
+    x[15] = float(P.psi);
     y[6] = float(eq1 - eq0);
 }
 

@@ -1,0 +1,524 @@
+// mpm_owned.hpp -- base-owned tile scatters: each lane owns one stencil base.
+//
+// After a sort (mpm_sort.hpp) the particles of a tile of 4x4x4 stencil bases
+// (8x8 in 2D) are laid out in rank groups: group r holds the r-th particle of
+// every base that has more than r particles, in base order, and the groups of
+// ranks >= 31 are merged into one block per base.  One wavefront takes one
+// tile, and lane L owns base L of the tile: in round r it loads the r-th
+// particle of its base (a coalesced load, the group is contiguous) and adds
+// that particle's contributions to the S^D nodes of its stencil in REGISTERS.
+// Only after the last round does the lane add its sums into an LDS window of
+// the tile's nodes, one stencil slot at a time: at a fixed slot the 64 bases
+// of a tile reach 64 distinct nodes, so the wave's read-add-write of one slot
+// never collides, and the window holds each node's sum over the whole tile.
+// The window is then flushed with one global atomic per touched node and
+// field, as in the rank-round scatter (mpm_tiled.hpp).
+//
+// Against the rank-round scatter this moves the per-particle work from LDS
+// read-add-write (27 dependent slots of 7 fields per particle in 3D) to
+// register FMAs; LDS sees one read-add-write pass per base and tile.
+//
+// A particle that left its base since the sort cannot use its lane's
+// registers (their node offsets are those of the lane's base).  It goes to a
+// drift list in LDS; batches of 64 drifted particles claim their bases in a
+// small LDS table (write a unique tag, read it back), the winners add into
+// the window with the same lockstep read-add-write, and the rest claim again;
+// a drifted particle whose stencil left the window scatters to the grid with
+// global atomics.  Results never depend on the sort interval.
+//
+// The accumulators must fit the register file: 3D P2G #1 sums mass and
+// momentum in one pass over the tile's particles and the force in a second
+// (FP32), and FP64 3D also takes the z layers of the stencil one at a time
+// (OwnedPasses).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mpm_kernels.hpp"
+#include "mpm_tiled.hpp"
+
+namespace mpm {
+
+// Window read-add-write of one node: c[f] added to field f of window node k
+// (paired FP32 layout or one field per access, as kPairedP2G selects).
+template <int NF, typename R, int WN, bool PAIRED>
+__device__ __forceinline__ void window_add_node(lds_ptr<R> W, int k, const R* c) {
+    if constexpr (PAIRED) {
+        using u64p = volatile __attribute__((address_space(3))) unsigned long long*;
+        constexpr int NP = NF / 2;
+        unsigned long long o[NP];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) o[q] = *reinterpret_cast<u64p>(W + 2 * q * WN + 2 * k);
+        R o1 = R(0);
+        if constexpr (NF % 2) o1 = W[2 * NP * WN + k];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            const float a = __uint_as_float(unsigned(o[q])) + c[2 * q];
+            const float b = __uint_as_float(unsigned(o[q] >> 32)) + c[2 * q + 1];
+            *reinterpret_cast<u64p>(W + 2 * q * WN + 2 * k) =
+                (unsigned long long)__float_as_uint(a) | ((unsigned long long)__float_as_uint(b) << 32);
+        }
+        if constexpr (NF % 2) W[2 * NP * WN + k] = o1 + c[NF - 1];
+    } else {
+        R o[NF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) o[f] = W[f * WN + k];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) W[f * WN + k] = o[f] + c[f];
+    }
+}
+
+// Hides the values of a register array from the optimiser (an empty asm
+// that may change them), so computations on them stay where they are written.
+template <typename T, int N>
+__device__ __forceinline__ void opaque_regs(T (&a)[N]) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) asm volatile("" : "+v"(a[k]));
+}
+template <typename T, int N, int M>
+__device__ __forceinline__ void opaque_regs(T (&a)[N][M]) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) opaque_regs(a[k]);
+}
+
+// Particle data a scatter reads.
+template <typename R>
+struct ScatterIn {
+    double x[3];
+    R m, vol, v[3], sg[6];
+};
+
+template <int DIM, typename R, int MODE>
+__device__ __forceinline__ void load_scatter_in(const PartPtrs<R>& pp, int i, ScatterIn<R>& P) {
+    load_x<DIM>(pp.x, i, P.x);
+    if constexpr (MODE == 0) {
+        P.m = pp.ms[i];
+        P.vol = pp.vol[i];
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) P.v[a] = pp.v[a][i];
+        P.sg[0] = pp.sig[0][i];
+        P.sg[1] = pp.sig[1][i];
+        P.sg[3] = pp.sig[3][i];
+        if constexpr (DIM == 3) {
+            P.sg[2] = pp.sig[2][i];
+            P.sg[4] = pp.sig[4][i];
+            P.sg[5] = pp.sig[5][i];
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) P.v[a] = pp.pm[a][i];
+    }
+}
+
+// Passes over a tile's particles.  SPLIT 0: one pass, every field and z
+// layer.  SPLIT 1 (3D P2G #1): mass and momentum, then force.  SPLIT 2: the
+// field groups of SPLIT 1, each one z layer of the stencil at a time.  A
+// pass p sums fields [F0, F1) on z layers [Z0, Z0 + NZP) in registers.
+template <int DIM, int S, int MODE, int SPLIT>
+struct OwnedPasses {
+    static constexpr int NF = MODE == 0 ? 1 + 2 * DIM : DIM;
+    static constexpr int NZ = DIM == 3 ? S : 1;
+    static constexpr int NG = (MODE == 0 && DIM == 3 && SPLIT >= 1) ? 2 : 1;  // field groups
+    static constexpr int NL = (DIM == 3 && SPLIT == 2) ? NZ : 1;              // layer groups
+    static constexpr int NPASS = NG * NL;
+    static constexpr int NZP = NZ / NL;
+    static constexpr int F0(int p) { return NG == 1 ? 0 : (p / NL == 0 ? 0 : 1 + DIM); }
+    static constexpr int F1(int p) { return NG == 1 ? NF : (p / NL == 0 ? 1 + DIM : NF); }
+    static constexpr int Z0(int p) { return (p % NL) * NZP; }
+    static constexpr int MAXF = NG == 1 ? NF : (1 + DIM > DIM ? 1 + DIM : DIM);
+};
+
+// Register accumulators of one pass: NFP fields x NZP z layers x S rows x S
+// columns (2D: NZP = 1, the row is y).
+template <int NFP, int NZP, int S, typename R>
+struct OwnedAcc {
+    R v[NFP][NZP][S][S];
+};
+
+// Adds one particle's contributions to fields [F0, F1) on the z layers
+// [Z0, Z0 + NZP) of the lane's accumulators.  MODE 0 is P2G #1 (P2G_OS!,
+// utils_OS.jl:424-496): mass, momentum and internal + external force, with
+// the force factored per (y, z) node pair as in p2g3_window: with A = Ny Nz,
+// B = dNy Nz, C = Ny dNz the gradient is (dNx A, Nx B, Nx C), so
+// f_k = dNx P_k + Nx Q_k, P = -Ω σ_{kx} A, Q_k = -Ω (σ_{ky} B + σ_{kz} C) + m A g_k.
+// MODE 1 is the re-map momentum scatter (doublemapping2_OS!,
+// utils_OS.jl:917-959): N pm.  The stencil's weights are zero for a lane
+// that sits the round out, so the sums are branch-free.
+template <int DIM, int S, typename R, int MODE, int F0, int F1, int Z0, int NZP>
+__device__ __forceinline__ void owned_accumulate(const Stencil<DIM, S, R>& st, const ScatterIn<R>& P,
+                                                 const Gravity<R>& G, OwnedAcc<F1 - F0, NZP, S, R>& acc) {
+    auto& s = acc.v;
+    if constexpr (MODE == 0) {
+        constexpr bool MOM = F0 == 0;          // mass and momentum in this pass
+        constexpr bool FRC = F1 == 1 + 2 * DIM;  // force in this pass
+        const R m = P.m;
+        const R mv = -P.vol;
+        R pv[3] = {0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) pv[d] = m * P.v[d];
+        // -Ω σ
+        const R sxx = mv * P.sg[0], syy = mv * P.sg[1], sxy = mv * P.sg[3];
+        const R szz = DIM == 3 ? mv * P.sg[2] : R(0);
+        const R syz = DIM == 3 ? mv * P.sg[4] : R(0);
+        const R sxz = DIM == 3 ? mv * P.sg[5] : R(0);
+        R mNx[S], pN[S][3];
+        static_for<S>([&](auto a) {
+            mNx[a] = m * st.N[0][a];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) pN[a][d] = pv[d] * st.N[0][a];
+        });
+        static_for<NZP>([&](auto cc) {
+            constexpr int c = Z0 + decltype(cc)::value;
+            static_for<S>([&](auto b) {
+                R A, B, Cz;
+                if constexpr (DIM == 3) {
+                    A = st.N[1][b] * st.N[2][c];
+                    B = st.dN[1][b] * st.N[2][c];
+                    Cz = st.N[1][b] * st.dN[2][c];
+                } else {
+                    A = st.N[1][b];
+                    B = st.dN[1][b];
+                    Cz = R(0);
+                }
+                R Pf[3] = {0, 0, 0}, Qf[3] = {0, 0, 0};
+                if constexpr (FRC) {
+                    const R mA = m * A;
+                    if constexpr (DIM == 3) {
+                        Pf[0] = sxx * A; Pf[1] = sxy * A; Pf[2] = sxz * A;
+                        Qf[0] = sxy * B + sxz * Cz + mA * G.g[0];
+                        Qf[1] = syy * B + syz * Cz + mA * G.g[1];
+                        Qf[2] = syz * B + szz * Cz + mA * G.g[2];
+                    } else {
+                        Pf[0] = sxx * A; Pf[1] = sxy * A;
+                        Qf[0] = sxy * B + mA * G.g[0];
+                        Qf[1] = syy * B + mA * G.g[1];
+                    }
+                }
+                static_for<S>([&](auto a) {
+                    if constexpr (MOM) {
+                        s[0][cc][b][a] += mNx[a] * A;
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) s[1 + d][cc][b][a] += pN[a][d] * A;
+                    }
+                    if constexpr (FRC) {
+                        constexpr int o = 1 + DIM - F0;
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d)
+                            s[o + d][cc][b][a] += st.dN[0][a] * Pf[d] + st.N[0][a] * Qf[d];
+                    }
+                });
+            });
+        });
+    } else {
+        R pN[S][3];
+        static_for<S>([&](auto a) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) pN[a][d] = P.v[d] * st.N[0][a];
+        });
+        static_for<NZP>([&](auto cc) {
+            constexpr int c = Z0 + decltype(cc)::value;
+            static_for<S>([&](auto b) {
+                const R A = DIM == 3 ? st.N[1][b] * st.N[2][c] : st.N[1][b];
+                static_for<S>([&](auto a) {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) s[d][cc][b][a] += pN[a][d] * A;
+                });
+            });
+        });
+    }
+}
+
+// Window read-add-write of fields [F0, F1) of node k (c[f - F0]); a pair of
+// the paired layout with one field in the range is accessed as one float.
+template <int NF, int F0, int F1, typename R, int WN, bool PAIRED>
+__device__ __forceinline__ void window_add_fields(lds_ptr<R> W, int k, const R* c) {
+    if constexpr (PAIRED && F0 == 0 && F1 == NF) {
+        window_add_node<NF, R, WN, true>(W, k, c);
+    } else {
+        constexpr int NFP = F1 - F0;
+        R o[NFP];
+        static_for<NFP>([&](auto q) {
+            constexpr int f = F0 + decltype(q)::value;
+            o[q] = W[PAIRED ? paired_offset<NF>(f, k, WN) : f * WN + k];
+        });
+        static_for<NFP>([&](auto q) {
+            constexpr int f = F0 + decltype(q)::value;
+            W[PAIRED ? paired_offset<NF>(f, k, WN) : f * WN + k] = o[q] + c[q];
+        });
+    }
+}
+
+// Contribution of one particle to one node (drifted particles).
+template <int DIM, typename R, int MODE>
+__device__ __forceinline__ void owned_node_values(R N, R dNx, R dNy, R dNz, const ScatterIn<R>& P,
+                                                  const Gravity<R>& G, R* c) {
+    if constexpr (MODE == 0) {
+        R pv[3] = {0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) pv[d] = P.m * P.v[d];
+        p2g_values<DIM, R>(N, dNx, dNy, dNz, P.m, P.vol, pv, P.sg, G, c);
+    } else {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) c[d] = P.v[d] * N;
+    }
+}
+
+// Stencil loop of the drift paths: a runtime loop over the S^(D-1) (y, z)
+// rows with x unrolled (the rare drifted particles need few registers more
+// than they need speed); body(row offset, column a, N, dNx, dNy, dNz) in the
+// factor order of for_stencil.
+template <int DIM, int S, typename R, typename F>
+__device__ __forceinline__ void for_stencil_rows(const Stencil<DIM, S, R>& st, F&& body) {
+    constexpr int NROW = DIM == 3 ? S * S : S;
+#pragma unroll 1
+    for (int q = 0; q < NROW; ++q) {
+        const int b = q % S, c = DIM == 3 ? q / S : 0;
+        R Ny = st.N[1][0], dNy = st.dN[1][0], Nz = R(1), dNz = R(0);
+        static_for<S>([&](auto k) {
+            if (b == int(k)) {
+                Ny = st.N[1][k];
+                dNy = st.dN[1][k];
+            }
+            if (DIM == 3 && c == int(k)) {
+                Nz = st.N[2][k];
+                dNz = st.dN[2][k];
+            }
+        });
+        static_for<S>([&](auto a) {
+            const R Nx = st.N[0][a];
+            R N, dNx, dNyv, dNzv;
+            if constexpr (DIM == 3) {
+                N = Nx * Ny * Nz;
+                dNx = st.dN[0][a] * Ny * Nz;
+                dNyv = dNy * Nx * Nz;
+                dNzv = dNz * Nx * Ny;
+            } else {
+                N = Nx * Ny;
+                dNx = st.dN[0][a] * Ny;
+                dNyv = dNy * Nx;
+                dNzv = R(0);
+            }
+            body(b, c, int(a), N, dNx, dNyv, dNzv);
+            __builtin_amdgcn_sched_barrier(0);
+        });
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The scatter kernel.  MODE 0: P2G #1; MODE 1: the MUSL re-map momentum
+// scatter (from pp.pm).  SPLIT: passes over the tile (OwnedPasses).
+// ---------------------------------------------------------------------------
+template <int DIM, int S, int TSH, typename R, int MODE, int SPLIT>
+__global__ __launch_bounds__(64) void k_scatter_owned(
+    GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, Gravity<R> G,
+    const unsigned long long* __restrict__ masks, const int* __restrict__ offs,
+    const int* __restrict__ ovf, const int* __restrict__ rounds,
+    const int* __restrict__ tile_list, const int* __restrict__ seg_begin,
+    const int* __restrict__ seg_end, const int* __restrict__ count_ptr, int* flags) {
+    using W = Win<DIM, S, TSH>;
+    using PS = OwnedPasses<DIM, S, MODE, SPLIT>;
+    static_assert(kWinSlack == 0, "lane-owned bases assume a slack-free window");
+    static_assert(W::NBN == 64, "one base per lane");
+    constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
+    constexpr int NF = PS::NF;
+    constexpr bool PAIRED = kPairedP2G<DIM, NF, R, WN>;
+    constexpr int kLast = (1 << kRankBits) - 1;  // rank of the merged last group
+    __shared__ __attribute__((aligned(16))) R win_[NF * WN];
+    __shared__ unsigned claim_[64];
+    constexpr int kDriftCap = 256;  // drift list entries (LDS)
+    __shared__ int drift_[kDriftCap];
+    lds_ptr<R> win = (lds_ptr<R>)win_;
+    lds_ptr<unsigned> claim = (lds_ptr<unsigned>)claim_;
+    lds_ptr<int> drift = (lds_ptr<int>)drift_;
+    R* fields[NF];
+    if constexpr (MODE == 0) {
+        fields[0] = gp.m;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            fields[1 + d] = gp.p[d];
+            fields[1 + DIM + d] = gp.f[d];
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) fields[d] = gp.p[d];
+    }
+    const int lane = threadIdx.x;
+    const unsigned long long lt = (1ull << lane) - 1ull;  // lanes below this one
+    // the lane's base inside the tile (x fastest), and its node in the window
+    const int lx = lane & (NB - 1);
+    const int ly = (lane >> TSH) & (NB - 1);
+    const int lz = DIM == 3 ? lane >> (2 * TSH) : 0;
+    const int wbase0 = DIM == 3 ? lz * ZS + ly * WW + lx : ly * WW + lx;
+    unsigned tag = unsigned(lane);
+    int flag = 0;
+    int e0, e1, est;
+    xcd_chunk(*count_ptr, e0, e1, est);
+    for (int e = e0; e < e1; e += est) {
+        const int tile = tile_list[e];
+        const int beg = seg_begin[tile], end = seg_end[tile];
+        const int nr = rounds[tile];
+        int w0[3];
+        tile_window_origin<DIM>(tg, tile, w0);
+        const int myb[3] = {w0[0] + lx, w0[1] + ly, w0[2] + lz};
+        for (int k = lane; k < NF * WN; k += 64) win[k] = R(0);
+        // the merged last group: this base's particles beyond rank 30
+        const unsigned long long* tmask = masks + (long long)tile * kLast;
+        const int* toff = offs + (long long)tile * (kLast + 1);
+        int xbeg = 0, xcnt = 0;
+        if (nr > kLast) {
+            const int o = ovf[(long long)tile * 64 + lane];
+            const int o1 = lane < 63 ? ovf[(long long)tile * 64 + lane + 1] : end - beg - toff[kLast];
+            xbeg = beg + toff[kLast] + o;
+            xcnt = o1 - o;
+        }
+        // particle index of round r for this lane; false if the lane sits out
+        auto round_index = [&](int r, int& i) -> bool {
+            if (r < kLast) {
+                const unsigned long long mk = tmask[r];
+                i = beg + toff[r] + __popcll(mk & lt);
+                return (mk >> lane) & 1ull;
+            }
+            i = xbeg + (r - kLast);
+            return r - kLast < xcnt;
+        };
+        int ndrift = 0;
+        // Drifted particles drift_[k0, k0 + n), n <= 64: claim rounds into the
+        // window, global atomics outside it.  Every lane runs this (wave-uniform n).
+        auto drift_batch = [&](int k0, int n) {
+            ScatterIn<R> P;
+            Stencil<DIM, S, R> st;
+            bool pend = false;
+            int node0 = 0, bidx = 0;
+            if (lane < n) {
+                const int i = drift[k0 + lane];
+                load_scatter_in<DIM, R, MODE>(pp, i, P);
+                int fl2 = 0;
+                if (make_stencil<DIM, S, R>(g, P.x, st, fl2)) {
+                    const bool inside = window_slot<DIM, NB, WW, ZS>(st.base, w0, node0, bidx);
+                    if (inside) {
+                        pend = true;
+                    } else {
+                        for_stencil_rows<DIM, S, R>(st, [&](int b, int c, int a, R N, R dNx, R dNy, R dNz) {
+                            if (N != R(0)) {
+                                R cv[NF];
+                                owned_node_values<DIM, R, MODE>(N, dNx, dNy, dNz, P, G, cv);
+                                const long long nn = node_id(g, st.base[0] + a, st.base[1] + b,
+                                                             DIM == 3 ? st.base[2] + c : 0);
+#pragma unroll
+                                for (int f = 0; f < NF; ++f) atomic_add(&fields[f][nn], cv[f]);
+                            }
+                        });
+                    }
+                }
+            }
+            while (__ballot(pend) != 0ull) {
+                bool won = false;
+                if (pend) {
+                    claim[bidx] = tag;
+                    won = claim[bidx] == tag;
+                }
+                tag += 64u;
+                if (won) {
+                    for_stencil_rows<DIM, S, R>(st, [&](int b, int c, int a, R N, R dNx, R dNy, R dNz) {
+                        if (N != R(0)) {
+                            R cv[NF];
+                            owned_node_values<DIM, R, MODE>(N, dNx, dNy, dNz, P, G, cv);
+                            window_add_node<NF, R, WN, PAIRED>(win, node0 + c * ZS + b * WW + a, cv);
+                        }
+                    });
+                    pend = false;
+                }
+            }
+        };
+        static_for<PS::NPASS>([&](auto pass) {
+            constexpr int p = decltype(pass)::value;
+            constexpr int F0 = PS::F0(p), F1 = PS::F1(p), Z0 = PS::Z0(p), NZP = PS::NZP;
+            constexpr bool FIRST = p == 0;
+            using Acc = OwnedAcc<F1 - F0, NZP, S, R>;
+            auto zero_sums = [](Acc& acc) {
+#pragma unroll
+                for (int f = 0; f < F1 - F0; ++f)
+#pragma unroll
+                    for (int cc = 0; cc < NZP; ++cc)
+#pragma unroll
+                        for (int b = 0; b < S; ++b)
+#pragma unroll
+                            for (int a = 0; a < S; ++a) acc.v[f][cc][b][a] = R(0);
+            };
+            // The lane's sums into the window, one stencil slot at a time (the
+            // 64 bases reach 64 distinct nodes at each slot).  The empty asm
+            // keeps the compiler from holding 27 slot addresses across the rounds.
+            auto add_sums = [&](const Acc& acc, bool any) {
+                int wbase = wbase0;
+                asm volatile("" : "+v"(wbase));
+                if (!any) return;
+                static_for<NZP>([&](auto cc) {
+                    static_for<S>([&](auto b) {
+                        static_for<S>([&](auto a) {
+                            constexpr int C = Z0 + decltype(cc)::value;
+                            const int k = wbase + (DIM == 3 ? C * ZS : 0) + int(b) * WW + int(a);
+                            R c[F1 - F0];
+#pragma unroll
+                            for (int f = 0; f < F1 - F0; ++f) c[f] = acc.v[f][cc][b][a];
+                            window_add_fields<NF, F0, F1, R, WN, PAIRED>(win, k, c);
+                            __builtin_amdgcn_sched_barrier(0);
+                        });
+                    });
+                });
+            };
+            Acc acc;
+            zero_sums(acc);
+            bool any = false;  // the lane's base holds a particle
+            for (int r = 0; r < nr; ++r) {
+                int i = 0;
+                const bool act = round_index(r, i);
+                // every lane loads (a lane that sits out reads the tile's first
+                // particle) and sums with zero weights unless its particle is home
+                ScatterIn<R> P;
+                load_scatter_in<DIM, R, MODE>(pp, act ? i : beg, P);
+                Stencil<DIM, S, R> st;
+                int fl2 = 0;
+                const bool ok = make_stencil<DIM, S, R>(g, P.x, st, fl2);
+                if (FIRST && MODE == 0 && act) flag |= fl2;
+                const bool home = act && ok && st.base[0] == myb[0] && st.base[1] == myb[1] &&
+                                  (DIM == 2 || st.base[2] == myb[2]);
+                any = any || home;
+                // The reference skips nodes with Ni == 0 (utils_OS.jl:431,470); a
+                // zero axis weight zeroes every node of that row, so zeroing its
+                // derivative (non-zero for the linear basis on a node) skips them.
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+#pragma unroll
+                    for (int k = 0; k < S; ++k) {
+                        const R n = home ? st.N[d][k] : R(0);
+                        st.dN[d][k] = n == R(0) ? R(0) : st.dN[d][k];
+                        st.N[d][k] = n;
+                    }
+                owned_accumulate<DIM, S, R, MODE, F0, F1, Z0, NZP>(st, P, G, acc);
+                if constexpr (FIRST) {
+                    const bool drifted = act && ok && !home;
+                    const unsigned long long dm = __ballot(drifted);
+                    if (dm != 0ull) {
+                        if (drifted) drift[ndrift + __popcll(dm & lt)] = i;
+                        ndrift += __popcll(dm);
+                        if (ndrift > kDriftCap - 64) {
+                            // The list is nearly full.  The sums so far go to the
+                            // window first, so the batch does not hold registers
+                            // beside them.
+                            add_sums(acc, any);
+                            zero_sums(acc);
+                            any = false;
+                            for (int k0 = 0; k0 < ndrift; k0 += 64) drift_batch(k0, min(64, ndrift - k0));
+                            ndrift = 0;
+                        }
+                    }
+                }
+            }
+            add_sums(acc, any);
+        });
+        for (int k0 = 0; k0 < ndrift; k0 += 64) drift_batch(k0, min(64, ndrift - k0));
+        flush_window<DIM, WW, ZS, WN, NF, R>(g, w0, win, fields);
+    }
+    raise_flag(flags, flag);
+}
+
+}  // namespace mpm

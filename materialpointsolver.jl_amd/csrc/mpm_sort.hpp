@@ -85,7 +85,8 @@ template <int DIM>
 __global__ __launch_bounds__(64) void k_tile_layout(SortKeyGeom kg, const unsigned* __restrict__ count,
                                                     unsigned long long* __restrict__ masks,
                                                     int* __restrict__ offs, int* __restrict__ ovf,
-                                                    int* __restrict__ totals, int ntiles) {
+                                                    int* __restrict__ totals, int* __restrict__ rounds,
+                                                    int ntiles) {
     const int t = blockIdx.x, lane = threadIdx.x;
     if (t >= ntiles) return;
     const int tm = (1 << kg.tsh) - 1;
@@ -114,9 +115,14 @@ __global__ __launch_bounds__(64) void k_tile_layout(SortKeyGeom kg, const unsign
     }
     ovf[(long long)t * 64 + lane] = incl - o;
     const int tail = __shfl(incl, 63);
+    // rounds of the base-owned scatter: the largest particle count of a base
+    int cmax = c;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) cmax = max(cmax, __shfl_xor(cmax, d));
     if (lane == 0) {
         offs[(long long)t * (kLastRank + 1) + kLastRank] = acc;
         totals[t] = acc + tail;
+        rounds[t] = cmax;
     }
 }
 
@@ -209,6 +215,7 @@ struct SortState {
     // counting-sort layout per tile (k_tile_layout)
     unsigned long long* masks = nullptr;
     int *offs = nullptr, *ovf = nullptr, *totals = nullptr, *starts = nullptr;
+    int* rounds = nullptr;  // per tile: the largest particle count of one base
     void* temp = nullptr;
     size_t temp_bytes = 0;
     // Alternate buffers, one per permuted stream in for_each_stream order;
@@ -260,6 +267,7 @@ struct SortState {
         if (e == hipSuccess) e = hipMalloc((void**)&ovf, sizeof(int) * nt * 64);
         if (e == hipSuccess) e = hipMalloc((void**)&totals, sizeof(int) * nt);
         if (e == hipSuccess) e = hipMalloc((void**)&starts, sizeof(int) * nt);
+        if (e == hipSuccess) e = hipMalloc((void**)&rounds, sizeof(int) * nt);
         if (e != hipSuccess) {
             error = std::string("hipMalloc: ") + hipGetErrorString(e);
             return MPM_E_NOMEM;
@@ -281,13 +289,13 @@ struct SortState {
 
     void release() {
         void* ps[] = {keys, keys_alt, idx, idx_alt, temp, seg_begin, seg_end, tile_list,
-                      tile_count, nonempty, count, masks, offs, ovf, totals, starts};
+                      tile_count, nonempty, count, masks, offs, ovf, totals, starts, rounds};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         keys = keys_alt = count = nullptr;
         idx = idx_alt = nullptr;
         masks = nullptr;
-        offs = ovf = totals = starts = nullptr;
+        offs = ovf = totals = starts = rounds = nullptr;
         temp = nullptr;
         seg_begin = seg_end = tile_list = tile_count = nullptr;
         nonempty = nullptr;
@@ -327,11 +335,11 @@ struct SortState {
             if (dim == 3) {
                 if (S == 3) k_sort_keys<3, 3><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
                 else k_sort_keys<3, 2><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
-                k_tile_layout<3><<<unsigned(tg.ntiles), 64, 0, s>>>(kg, count, masks, offs, ovf, totals, tg.ntiles);
+                k_tile_layout<3><<<unsigned(tg.ntiles), 64, 0, s>>>(kg, count, masks, offs, ovf, totals, rounds, tg.ntiles);
             } else {
                 if (S == 3) k_sort_keys<2, 3><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
                 else k_sort_keys<2, 2><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
-                k_tile_layout<2><<<unsigned(tg.ntiles), 64, 0, s>>>(kg, count, masks, offs, ovf, totals, tg.ntiles);
+                k_tile_layout<2><<<unsigned(tg.ntiles), 64, 0, s>>>(kg, count, masks, offs, ovf, totals, rounds, tg.ntiles);
             }
             size_t bytes = temp_bytes;
             lib = hipcub::DeviceScan::ExclusiveSum(temp, bytes, totals, starts, tg.ntiles, s);

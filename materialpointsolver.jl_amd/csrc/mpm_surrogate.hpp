@@ -30,10 +30,11 @@ struct SurrogateIn {
             for (int k = 0; k < 4; ++k) f[k] = pp.de[2 + k][p];
             const Material<float>& M = mats[pp.nid[p]];
             const float eq = pp.eq[p];
+            const MCStrength<float> P = mc_strength<float>(M, eq);
             f[4] = eq;
-            f[5] = M.phi;
-            f[6] = mc_cohesion<float>(M, eq);
-            f[7] = M.psi;
+            f[5] = P.phi;
+            f[6] = P.c;
+            f[7] = P.psi;
         }
     }
 };
@@ -56,10 +57,11 @@ struct SurrogateOut {
         for (int k = 0; k < 6; ++k) s[k] = pp.sig[k][p];
         const Material<float> M = mats[pp.nid[p]];
         float eq = pp.eq[p], ek = 0.0f;
-        const float c = mc_cohesion<float>(M, eq);
+        const MCStrength<float> P = mc_strength<float>(M, eq);
+        const float c = P.c;
         float lam[3], V[3][3];
         sym3_eig<float>(s, lam, V);
-        const float f = (lam[0] - lam[2]) + (lam[0] + lam[2]) * M.sinphi - 2.0f * c * M.cosphi;
+        const float f = (lam[0] - lam[2]) + (lam[0] + lam[2]) * P.sinphi - 2.0f * c * P.cosphi;
         const bool plastic = f > 0.0f;
         bool fallback = false;
         if (plastic) {
@@ -68,8 +70,8 @@ struct SurrogateOut {
             for (int k = 0; k < 6; ++k) sp[k] = s[k] + y[k];
             float lp[3], Vp[3][3];
             sym3_eig<float>(sp, lp, Vp);
-            const float F = (lp[0] - lp[2]) + (lp[0] + lp[2]) * M.sinphi - 2.0f * c * M.cosphi;
-            float scale = 2.0f * c * M.cosphi + fabsf(lp[0] + lp[2]) * M.sinphi;
+            const float F = (lp[0] - lp[2]) + (lp[0] + lp[2]) * P.sinphi - 2.0f * c * P.cosphi;
+            float scale = 2.0f * c * P.cosphi + fabsf(lp[0] + lp[2]) * P.sinphi;
             scale = fmaxf(scale, 1e-6f * (fabsf(lp[0]) + fabsf(lp[2])) + 1e-30f);
             const bool ok = fabsf(F) <= tol * scale && y[6] >= 0.0f && isfinite(F) &&
                             isfinite(y[6]);

@@ -109,9 +109,9 @@ function host2device(
         f64(mp.Ω0), f64(mp.ρs), f64(mp.ρs0), f64(mp.ϵq), f64(mp.ϵk), i64(attr.nid)))
     check(p, ccall((:mpm_upload_materials, LIB), Cint,
         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
-         Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+         Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
         p, f64(attr.Ks), f64(attr.Gs), f64(attr.ϕ), f64(attr.ψ), f64(attr.c), f64(attr.σt),
-        f64(attr.Hp), f64(attr.cr)))
+        f64(attr.Hp), f64(attr.cr), f64(attr.ϕr)))
     vz_idx = dim == 3 ? i64(bc.vz_s_idx) : C_NULL
     vz_val = dim == 3 ? f64(bc.vz_s_val) : C_NULL
     check(p, ccall((:mpm_upload_bc, LIB), Cint,

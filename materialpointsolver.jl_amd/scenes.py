@@ -44,7 +44,7 @@ def _lattice(lo, hi, step):
 def column_collapse_2d(*, h=0.0025, width=0.2, height=0.1, box_x=0.5, box_y=0.12, pad=2,
                        dt=2e-5, Ttol=None, nsteps=50_000, eps="FP64", constitutive="druckerprager",
                        phi_deg=PHI_DEG, cohesion=0.0, psi_deg=0.0, sigma_t=0.0, Hp=0.0, cr=0.0,
-                       va="a", MVL=False, zeta=0.0, Te=0.0, gravity=-9.8, basis="uGIMP",
+                       phir_deg=0.0, va="a", MVL=False, zeta=0.0, Te=0.0, gravity=-9.8, basis="uGIMP",
                        scheme="MUSL", sort_interval=None, device="HIP", project_name="collapse2d",
                        project_path="."):
     """Plane-strain granular column collapse; returns (args, grid, mp, attr, bc)."""
@@ -62,8 +62,8 @@ def column_collapse_2d(*, h=0.0025, width=0.2, height=0.1, box_x=0.5, box_y=0.12
     pts = meshbuilder(_lattice(0.0, width, lp), _lattice(0.0, height, lp))
     mp = UserParticle2D(ϵ=eps, NIC=nic, dx=lp, dy=lp, ξ=pts, ρs=np.full(len(pts), RHO))
     attr = UserProperty(ϵ=eps, nid=np.ones(mp.np), ν=[NU], Es=[E], Gs=[GS], Ks=[KS],
-                        σt=[sigma_t], ϕ=[math.radians(phi_deg)], ϕr=[0], ψ=[math.radians(psi_deg)],
-                        c=[cohesion], cr=[cr], Hp=[Hp])
+                        σt=[sigma_t], ϕ=[math.radians(phi_deg)], ϕr=[math.radians(phir_deg)],
+                        ψ=[math.radians(psi_deg)], c=[cohesion], cr=[cr], Hp=[Hp])
     gx, gy = grid.ξ[:, 0], grid.ξ[:, 1]
     vx_idx = ((gx <= 0.0) | (gx >= box_x) | (gy <= 0.0)).astype(np.int64)
     vy_idx = (gy <= 0.0).astype(np.int64)
@@ -75,11 +75,14 @@ def column_collapse_2d(*, h=0.0025, width=0.2, height=0.1, box_x=0.5, box_y=0.12
 def column_collapse_3d(*, h=0.005, width=0.4, height=0.2, box_xy=1.2, box_z=0.25, pad=2,
                        dt=None, cfl=0.2, Ttol=None, nsteps=20_000, eps="FP32",
                        constitutive="druckerprager", phi_deg=PHI_DEG, cohesion=0.0, psi_deg=0.0,
-                       sigma_t=0.0, va="a", MVL=False, zeta=0.0, Te=0.0, gravity=-9.8,
+                       sigma_t=0.0, Hp=0.0, cr=0.0, phir_deg=0.0, va="a", MVL=False, zeta=0.0,
+                       Te=0.0, gravity=-9.8,
                        basis="uGIMP", scheme="MUSL", sort_interval=None, device="HIP",
                        project_name="collapse3d", project_path="."):
     """3D granular column collapse centred in x and y; returns (args, grid, mp, attr, bc).
-    Floor z <= 0 no-slip; walls at x, y = +-box_xy/2 slip (normal velocity fixed)."""
+    Floor z <= 0 no-slip; walls at x, y = +-box_xy/2 slip (normal velocity fixed).
+    Hp, cr and phir_deg are the Mohr-Coulomb softening parameters (cohesion
+    softening modulus and residual cohesion and friction)."""
     if dt is None:
         dt = cfl * h / p_wave_speed()
     if Ttol is None:
@@ -99,8 +102,8 @@ def column_collapse_3d(*, h=0.005, width=0.4, height=0.2, box_xy=1.2, box_z=0.25
     pts = meshbuilder(_lattice(-w2, w2, lp), _lattice(-w2, w2, lp), _lattice(0.0, height, lp))
     mp = UserParticle3D(ϵ=eps, NIC=nic, dx=lp, dy=lp, dz=lp, ξ=pts, ρs=np.full(len(pts), RHO))
     attr = UserProperty(ϵ=eps, nid=np.ones(mp.np), ν=[NU], Es=[E], Gs=[GS], Ks=[KS],
-                        σt=[sigma_t], ϕ=[math.radians(phi_deg)], ϕr=[0], ψ=[math.radians(psi_deg)],
-                        c=[cohesion], cr=[0], Hp=[0])
+                        σt=[sigma_t], ϕ=[math.radians(phi_deg)], ϕr=[math.radians(phir_deg)],
+                        ψ=[math.radians(psi_deg)], c=[cohesion], cr=[cr], Hp=[Hp])
     gx, gy, gz = grid.ξ[:, 0], grid.ξ[:, 1], grid.ξ[:, 2]
     floor = gz <= 0.0
     vx_idx = ((gx <= -half) | (gx >= half) | floor).astype(np.int64)

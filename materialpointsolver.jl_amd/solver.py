@@ -134,7 +134,8 @@ class HIPModel:
             eps_q=f64(mp.ϵq), eps_k=f64(mp.ϵk), nid=i64(attr.nid))
         self._upload_particles(arrs)
         self._nid = arrs["nid"]
-        m = [f64(a) for a in (attr.Ks, attr.Gs, attr.ϕ, attr.ψ, attr.c, attr.σt, attr.Hp, attr.cr)]
+        m = [f64(a) for a in (attr.Ks, attr.Gs, attr.ϕ, attr.ψ, attr.c, attr.σt, attr.Hp, attr.cr,
+                              attr.ϕr)]
         if any(a.size != int(attr.nmat) for a in m):
             raise ValueError(f"material arrays must have attr.nmat = {int(attr.nmat)} entries")
         self._check(self.lib.mpm_upload_materials(self.ctx, *[capi.dptr(a) for a in m]))
@@ -209,7 +210,8 @@ class HIPModel:
         return int(out[0])
 
     def set_option(self, option, value):
-        """Runtime option: "sort_interval" (>= 1), "tiles" (bool) or
+        """Runtime option: "sort_interval" (>= 1), "tiles" (bool), "scatter"
+        (the tile scatter form: "owned", "owned_layers" or "rounds") or
         "profile_kernels" (kernel names that profile_enable times; empty or
         None = every kernel)."""
         if option == "profile_kernels":
@@ -221,6 +223,11 @@ class HIPModel:
                     raise ValueError(f"unknown kernel {name!r}; kernels: {names}")
                 mask |= 1 << names.index(name)
             self._check(self.lib.mpm_set_option(self.ctx, capi.MPM_OPT_PROFILE_MASK, mask))
+            return
+        if option == "scatter":
+            if value not in capi.SCATTER:
+                raise ValueError(f"scatter form {value!r}; forms: {sorted(capi.SCATTER)}")
+            self._check(self.lib.mpm_set_option(self.ctx, capi.MPM_OPT_SCATTER, capi.SCATTER[value]))
             return
         code = {"sort_interval": capi.MPM_OPT_SORT_INTERVAL, "tiles": capi.MPM_OPT_TILES}[option]
         self._check(self.lib.mpm_set_option(self.ctx, code, int(value)))

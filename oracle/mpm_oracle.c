@@ -62,6 +62,7 @@ typedef struct oracle_model {
     /* ---- property (src/types/property.jl) ---- */
     const int64_t *nid;    /* np, 1-based */
     const double *Ks, *Gs, *c, *phi, *psi, *sigt, *Hp, *cr; /* nmat */
+    const double *phir;    /* nmat: residual friction (3D Mohr-Coulomb softening) */
     /* ---- velocity boundary (src/types/boundary.jl) ---- */
     const int64_t *vx_idx, *vy_idx, *vz_idx; /* ni, != 0 -> fixed */
     const double *vx_val, *vy_val, *vz_val;
@@ -844,11 +845,34 @@ int oracle_mc3_principal(double *p, double Dt, double Dd, double sphi, double cp
     return 4;
 }
 
+/* Strength parameters in effect at accumulated plastic strain eq, following
+ * the linear softening rule of mohr_coulomb.tcc:8-25 (one peak/residual
+ * plastic-strain pair for cohesion, friction and dilation; peak at eq = 0):
+ *   cohesion c(eq) = max(c + Hp eq, cr)  (the 2D mcP! rule, mohrcoulomb.jl:36);
+ *   when Hp < 0, c > cr and phir > 0, the residual strain is
+ *   eq_r = (cr - c) / Hp and, with t = min(eq / eq_r, 1), friction goes
+ *   linearly from phi to phir and dilation from psi to 0 (no dilation at the
+ *   residual state); otherwise phi and psi stay (the reference's default
+ *   phir = 0 leaves them constant). */
+void oracle_mc_soften(double eq, double phi, double psi, double coh, double Hp, double cr,
+                      double phir, double *phi_out, double *psi_out, double *c_out) {
+    *c_out = jl_max(coh + Hp * eq, cr);
+    *phi_out = phi;
+    *psi_out = psi;
+    if (Hp < 0.0 && coh > cr && phir > 0.0) {
+        const double eqr = (cr - coh) / Hp;
+        const double t = eq >= eqr ? 1.0 : eq / eqr;
+        *phi_out = phi + (phir - phi) * t;
+        *psi_out = psi - psi * t;
+    }
+}
+
 /* One particle: s[6] in/out ([xx,yy,zz,xy,yz,xz]), eq in/out, ek out.
  * Returns the return type of oracle_mc3_principal. */
-int oracle_mc3_point(double *s, double *eq, double *ek, double Ks, double Gs, double phi,
-                     double psi, double coh, double Hp, double cr) {
-    const double c = jl_max(coh + Hp * (*eq), cr);
+int oracle_mc3_point(double *s, double *eq, double *ek, double Ks, double Gs, double phi0,
+                     double psi0, double coh, double Hp, double cr, double phir) {
+    double phi, psi, c;
+    oracle_mc_soften(*eq, phi0, psi0, coh, Hp, cr, phir, &phi, &psi, &c);
     const double Dt = Ks + 1.333333 * Gs, Dd = Ks - 0.666667 * Gs;
     double lam[3], V[9];
     oracle_sym3_eig(s, lam, V);
@@ -894,7 +918,7 @@ void oracle_mcP(oracle_model *m) {
             for (int j = 0; j < 6; ++j) s[j] = COL(m->sig, np, ix, j);
             double eq = m->eps_q[ix], ek = m->eps_k[ix];
             oracle_mc3_point(s, &eq, &ek, m->Ks[k], m->Gs[k], m->phi[k], m->psi[k], m->c[k],
-                             m->Hp[k], m->cr[k]);
+                             m->Hp[k], m->cr[k], m->phir ? m->phir[k] : 0.0);
             for (int j = 0; j < 6; ++j) COL(m->sig, np, ix, j) = s[j];
             m->eps_q[ix] = eq;
             m->eps_k[ix] = ek;

@@ -74,7 +74,7 @@ class OracleModel(C.Structure):
         ("eps_v", _D), ("omega", _D), ("omega0", _D), ("ms", _D), ("rho", _D), ("rho0", _D),
         ("p2c", _I), ("p2n", _I), ("Nij", _D), ("dNx", _D), ("dNy", _D), ("dNz", _D),
         ("nid", _I), ("Ks", _D), ("Gs", _D), ("c", _D), ("phi", _D), ("psi", _D),
-        ("sigt", _D), ("Hp", _D), ("cr", _D),
+        ("sigt", _D), ("Hp", _D), ("cr", _D), ("phir", _D),
         ("vx_idx", _I), ("vy_idx", _I), ("vz_idx", _I), ("vx_val", _D), ("vy_val", _D),
         ("vz_val", _D),
         ("gravity_x", C.c_double), ("gravity_y", C.c_double), ("Ti_now", C.c_double),
@@ -121,8 +121,10 @@ def load():
         lib.oracle_sym3_eig.restype = None
         lib.oracle_mc3_principal.argtypes = [_D] + [C.c_double] * 6
         lib.oracle_mc3_principal.restype = C.c_int
-        lib.oracle_mc3_point.argtypes = [_D, _D, _D] + [C.c_double] * 7
+        lib.oracle_mc3_point.argtypes = [_D, _D, _D] + [C.c_double] * 8
         lib.oracle_mc3_point.restype = C.c_int
+        lib.oracle_mc_soften.argtypes = [C.c_double] * 7 + [_D] * 3
+        lib.oracle_mc_soften.restype = None
         lib.oracle_struct_size.restype = C.c_int
         if lib.oracle_struct_size() != C.sizeof(OracleModel):
             raise ImportError("oracle_model layout mismatch")
@@ -166,13 +168,20 @@ def mc3_principal(p3, Dt, Dd, sphi, cphi, spsi, c):
     return p, kind
 
 
-def mc3_point(s6, eq, Ks, Gs, phi, psi, c, Hp=0.0, cr=0.0):
+def mc3_point(s6, eq, Ks, Gs, phi, psi, c, Hp=0.0, cr=0.0, phir=0.0):
     """Build-defined 3D Mohr-Coulomb on one stress; returns (s, eq, ek, kind)."""
     s = np.ascontiguousarray(s6, dtype=np.float64).copy()
     q, k = C.c_double(eq), C.c_double(0.0)
     kind = load().oracle_mc3_point(s.ctypes.data_as(_D), C.byref(q), C.byref(k), Ks, Gs, phi,
-                                   psi, c, Hp, cr)
+                                   psi, c, Hp, cr, phir)
     return s, q.value, k.value, kind
+
+
+def mc_soften(eq, phi, psi, c, Hp, cr, phir):
+    """(phi, psi, c) in effect at plastic strain eq (3D Mohr-Coulomb softening)."""
+    o = [C.c_double(0.0) for _ in range(3)]
+    load().oracle_mc_soften(eq, phi, psi, c, Hp, cr, phir, *[C.byref(v) for v in o])
+    return tuple(v.value for v in o)
 
 
 _CONST = {"linearelastic": 0, "druckerprager": 1, "mohrcoulomb": 2}
@@ -204,7 +213,7 @@ class Oracle:
             p2n=np.zeros((n, nic), dtype=np.int64, order="F"), Nij=zf(n, nic), dNx=zf(n, nic),
             dNy=zf(n, nic), dNz=zf(n, nic) if D == 3 else zf(1, 1),
             nid=i(attr.nid), Ks=f(attr.Ks), Gs=f(attr.Gs), c=f(attr.c), phi=f(attr.ϕ),
-            psi=f(attr.ψ), sigt=f(attr.σt), Hp=f(attr.Hp), cr=f(attr.cr),
+            psi=f(attr.ψ), sigt=f(attr.σt), Hp=f(attr.Hp), cr=f(attr.cr), phir=f(attr.ϕr),
             vx_idx=i(bc.vx_s_idx), vy_idx=i(bc.vy_s_idx),
             vz_idx=i(bc.vz_s_idx) if D == 3 else np.zeros(1, dtype=np.int64),
             vx_val=f(bc.vx_s_val), vy_val=f(bc.vy_s_val),

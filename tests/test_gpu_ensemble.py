@@ -140,9 +140,10 @@ def test_autotune_on_the_device(mpm):
     try:
         sa = m.step_args(args)
         best, table, Ti = mpm.autotune(m, sa, float(args.ΔT), 0.0, sort_intervals=(5, 20),
-                                       steps=5, warm=2)
-        assert len(table) == 4 and best["sort_interval"] in (5, 20)
-        print("autotune:", best, [(k, t, f"{s * 1e6:.0f} us") for k, t, s in table], flush=True)
+                                       steps=5, warm=2, scatters=("owned", "owned_layers", "rounds"))
+        assert len(table) == 8 and best["sort_interval"] in (5, 20)
+        print("autotune:", best, [(k, t, sc, f"{s * 1e6:.0f} us") for k, t, sc, s in table],
+              flush=True)
         m.run(sa, float(args.ΔT), Ti, 5, graph=True)
         st = m.download()
         assert np.all(np.isfinite(st["xi"]))

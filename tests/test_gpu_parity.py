@@ -17,12 +17,16 @@ def _rel(a, b):
 
 
 def _compare(mpm, orc, scene, nsteps, tol, sort_interval=0, keys=("xi", "vs", "sigma", "eps_s",
-                                                                   "eps_q", "omega")):
+                                                                   "eps_q", "omega"),
+             scatter=None):
     args, grid, mp, attr, bc = scene
     print(f"case: dim={grid.dim} eps={mp.ϵ} np={mp.np} const={args.constitutive} va={args.va} "
-          f"MVL={args.MVL} NIC={grid.NIC} steps={nsteps} sort={sort_interval}", flush=True)
+          f"MVL={args.MVL} NIC={grid.NIC} steps={nsteps} sort={sort_interval} "
+          f"scatter={scatter or 'default'}", flush=True)
     model = mpm.HIPModel(grid, mp, attr, bc, sort_interval=sort_interval)
     try:
+        if scatter is not None:
+            model.set_option("scatter", scatter)
         sargs = model.step_args(args)
         model.run(sargs, float(args.ΔT), 0.0, nsteps)
         model.sync()
@@ -83,6 +87,48 @@ def test_fp64_3d_mohrcoulomb(mpm, orc, va, sort):
     # this column yields within the 100 steps
     _compare(mpm, orc, _c3(mpm, constitutive="mohrcoulomb", va=va, cohesion=200.0, psi_deg=5.0),
              100, 1e-9, sort_interval=sort)
+
+
+# The three tile-scatter forms (base-owned register sums, the same one z
+# layer per pass, rank rounds into an LDS window) against the oracle.  Sort
+# interval 5 keeps most particles on their sorted base; 400 sorts once at the
+# start, so over 200 collapse steps many particles drift to other bases, other
+# tiles' windows and out of every window (the drift lists and their global
+# atomic fallback carry them).
+@pytest.mark.parametrize("sort", [5, 400])
+@pytest.mark.parametrize("scatter", ["owned", "owned_layers", "rounds"])
+def test_fp64_scatter_forms(mpm, orc, scatter, sort):
+    _compare(mpm, orc, _c2(mpm, constitutive="druckerprager", cohesion=200.0), 200, 1e-9,
+             sort_interval=sort, scatter=scatter)
+    _compare(mpm, orc, _c3(mpm, constitutive="druckerprager"), 100, 1e-9, sort_interval=sort,
+             scatter=scatter)
+
+
+@pytest.mark.parametrize("scatter", ["owned", "owned_layers", "rounds"])
+def test_fp32_scatter_forms(mpm, orc, scatter):
+    tol = dict(xi=2e-4, vs=5e-4, sigma=2e-3, eps_s=5e-4, eps_q=2e-3, omega=1e-5)
+    _compare(mpm, orc, _c3(mpm, eps="FP32", constitutive="druckerprager"), 100, tol,
+             sort_interval=10, scatter=scatter)
+    _compare(mpm, orc, _c3(mpm, eps="FP32", constitutive="druckerprager"), 100, tol,
+             sort_interval=400, scatter=scatter)
+
+
+# Mohr-Coulomb softening.  2D: cohesion max(c + Hp ϵq, cr) (mohrcoulomb.jl:36;
+# the reference's 2D mcP! ignores ϕr).  3D: cohesion the same, friction
+# 19.8 -> 12 deg and dilation 5 -> 0 deg linearly over the plastic strain in
+# which cohesion softens (eqr = (cr - c) / Hp = 0.01; mohr_coulomb.tcc:8-25).
+# Both columns end with particles before, inside and past the softening range.
+@pytest.mark.parametrize("sort", [0, 5])
+def test_fp64_mohrcoulomb_softening(mpm, orc, sort):
+    soft = dict(constitutive="mohrcoulomb", cohesion=200.0, Hp=-1.8e4, cr=20.0, phir_deg=12.0,
+                psi_deg=5.0)
+    for sc, n in ((_c2(mpm, **soft), 200), (_c3(mpm, **soft), 100)):
+        got, want = _compare(mpm, orc, sc, n, 1e-9, sort_interval=sort)
+        eq = want["eps_q"]
+        inside = np.mean((eq > 1e-3) & (eq < 0.01))
+        print(f"  softening: {inside:.2f} of the particles inside the range, "
+              f"{np.mean(eq >= 0.01):.2f} past it", flush=True)
+        assert inside > 0.05 and np.mean(eq >= 0.01) > 0.005
 
 
 # USL and USF schedules (OS_USL.jl, OS_USF.jl); PIC = 0.3 so that the USL :a

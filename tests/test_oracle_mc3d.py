@@ -200,3 +200,55 @@ def test_main_plane_strain_bookkeeping(orc):
     o_soft, _, _, _ = orc.mc3_point(s, 0.5, K, G, phi, psi, c, Hp=-1e3, cr=100.0)
     o_hard, _, _, _ = orc.mc3_point(s, 0.0, K, G, phi, psi, 500.0)
     assert np.allclose(o_soft, o_hard, rtol=1e-12)
+
+
+# --- softening (mohr_coulomb.tcc:8-25 linear rule; one peak/residual strain pair) ---
+PHI, PHIR, PSI = math.radians(30.0), math.radians(20.0), math.radians(6.0)
+COH, CR, HP = 10.0e3, 2.0e3, -400.0e3  # residual plastic strain (CR - COH) / HP = 0.02
+
+
+@pytest.mark.parametrize("eq, phi_deg, psi_deg, c", [
+    (0.0, 30.0, 6.0, 10.0e3),      # peak
+    (0.005, 27.5, 4.5, 8.0e3),     # a quarter of the way: t = 0.25
+    (0.01, 25.0, 3.0, 6.0e3),      # half way
+    (0.02, 20.0, 0.0, 2.0e3),      # residual reached
+    (0.05, 20.0, 0.0, 2.0e3),      # past it: residual values stay
+])
+def test_softening_rule_hand_values(orc, eq, phi_deg, psi_deg, c):
+    phi, psi, coh = orc.mc_soften(eq, PHI, PSI, COH, HP, CR, PHIR)
+    assert math.degrees(phi) == pytest.approx(phi_deg, abs=1e-12)
+    assert math.degrees(psi) == pytest.approx(psi_deg, abs=1e-12)
+    assert coh == pytest.approx(c, rel=1e-14)
+
+
+@pytest.mark.parametrize("Hp, cr, phir", [
+    (0.0, 2.0e3, PHIR),        # no cohesion softening modulus
+    (400.0e3, 2.0e3, PHIR),    # hardening cohesion
+    (HP, 10.0e3, PHIR),        # cohesion already residual: no strain interval
+    (HP, 2.0e3, 0.0),          # the reference's default phir = 0: friction stays
+])
+def test_softening_off_keeps_phi_and_psi(orc, Hp, cr, phir):
+    for eq in (0.0, 0.01, 0.1):
+        phi, psi, coh = orc.mc_soften(eq, PHI, PSI, COH, Hp, cr, phir)
+        assert (phi, psi) == (PHI, PSI)
+        assert coh == max(COH + Hp * eq, cr)  # the 2D mcP! cohesion rule
+
+
+@pytest.mark.parametrize("eq", [0.0, 0.004, 0.013, 0.03])
+def test_softened_return_is_the_return_with_softened_parameters(orc, eq):
+    # the softening only sets the strength the return uses, from the step-start
+    # plastic strain: the same return with those parameters fixed gives the
+    # same stress and strains
+    rng = np.random.default_rng(11)
+    phi_s, psi_s, c_s = orc.mc_soften(eq, PHI, PSI, COH, HP, CR, PHIR)
+    for _ in range(20):
+        s = rng.normal(scale=4.0e4, size=6) - np.array([3e4, 3e4, 3e4, 0, 0, 0])
+        a = orc.mc3_point(s, eq, K, G, PHI, PSI, COH, Hp=HP, cr=CR, phir=PHIR)
+        b = orc.mc3_point(s, eq, K, G, phi_s, psi_s, c_s, Hp=0.0, cr=c_s, phir=0.0)
+        np.testing.assert_array_equal(a[0], b[0])
+        assert a[1:] == b[1:]
+        if a[3] != 0:
+            # on the softened surface: max over pairs of the Mohr-Coulomb function
+            lam = np.linalg.eigvalsh(_tensor(a[0]))
+            f = (lam[2] - lam[0]) + (lam[2] + lam[0]) * math.sin(phi_s) - 2 * c_s * math.cos(phi_s)
+            assert abs(f) <= 1e-7 * (abs(lam).max() + c_s)
