@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define MPM_ABI_VERSION 1
+#define MPM_ABI_VERSION 2
 
 enum {
     MPM_OK = 0,
@@ -175,8 +175,14 @@ int mpm_run_graph(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int6
  *                         MPM_SCATTER_OWNED_LAYERS: the same, one z layer of
  *                         the stencil per pass (fewer registers, more waves);
  *                         MPM_SCATTER_ROUNDS: rank rounds of distinct bases
- *                         adding into an LDS window. */
-enum { MPM_OPT_SORT_INTERVAL = 0, MPM_OPT_TILES = 1, MPM_OPT_PROFILE_MASK = 2, MPM_OPT_SCATTER = 3 };
+ *                         adding into an LDS window.
+ *  MPM_OPT_FUSED_REMAP:   1 (default): with a base-owned scatter form, the MUSL
+ *                         re-map gathers, moves the particles and sums the new
+ *                         momentum in one kernel (not FP64 3D uGIMP, whose sums
+ *                         exceed the register file); 0: a gather kernel, then
+ *                         the momentum scatter. */
+enum { MPM_OPT_SORT_INTERVAL = 0, MPM_OPT_TILES = 1, MPM_OPT_PROFILE_MASK = 2, MPM_OPT_SCATTER = 3,
+       MPM_OPT_FUSED_REMAP = 4 };
 enum { MPM_SCATTER_ROUNDS = 0, MPM_SCATTER_OWNED = 1, MPM_SCATTER_OWNED_LAYERS = 2 };
 int mpm_set_option(mpm_ctx *ctx, int option, int64_t value);
 
@@ -216,6 +222,7 @@ int mpm_profile_kernel_count(void);
  * step's algorithmic byte count includes).  Diagnostic; synchronises.
  * No reference counterpart. */
 int mpm_count_active_nodes(mpm_ctx *ctx, int64_t *count);
+
 
 /* Triad bandwidth probe A = B + C on nbytes-per-array FP32 buffers, in GB/s
  * (10^9 B/s), best of `reps`.  Replaces Tpeak(Val(:ROCm))

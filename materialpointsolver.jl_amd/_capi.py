@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmpm_hip.so")
 
 MPM_OK = 0
+MPM_ABI_VERSION = 2  # mpm_upload_materials takes phir (version 1: without it)
 ERRORS = {-1: "MPM_E_ARG", -2: "MPM_E_UNSUPPORTED", -3: "MPM_E_HIP", -4: "MPM_E_NOMEM",
           -5: "MPM_E_STATE", -6: "MPM_E_RANGE"}
 MPM_FP64, MPM_FP32 = 0, 1
@@ -23,6 +24,7 @@ MPM_MUSL, MPM_USL, MPM_USF = 0, 1, 2
 SCHEME = {"MUSL": MPM_MUSL, "USL": MPM_USL, "USF": MPM_USF}
 MPM_VA_A, MPM_VA_V = 0, 1
 MPM_OPT_SORT_INTERVAL, MPM_OPT_TILES, MPM_OPT_PROFILE_MASK, MPM_OPT_SCATTER = 0, 1, 2, 3
+MPM_OPT_FUSED_REMAP = 4
 MPM_SCATTER_ROUNDS, MPM_SCATTER_OWNED, MPM_SCATTER_OWNED_LAYERS = 0, 1, 2
 SCATTER = {"rounds": MPM_SCATTER_ROUNDS, "owned": MPM_SCATTER_OWNED,
            "owned_layers": MPM_SCATTER_OWNED_LAYERS}
@@ -112,7 +114,7 @@ def load(path: str | None = None):
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
-    if lib.mpm_abi_version() != 1:
+    if lib.mpm_abi_version() != MPM_ABI_VERSION:
         raise ImportError("libmpm_hip.so ABI version mismatch")
     if path is None:
         _lib = lib

@@ -68,6 +68,7 @@ struct mpm_ctx {
     uint64_t prof_mask = 0;  // kernel ids to time; 0 = all
     bool use_tiles = true;  // LDS-binned scatters when the particles are tile-sorted
     int scatter = MPM_SCATTER_OWNED;  // tile scatter form (MPM_OPT_SCATTER)
+    bool fused_remap = true;          // one-pass MUSL re-map (MPM_OPT_FUSED_REMAP)
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> pool;
     // captured step graphs of mpm_run_graph, keyed by what their kernel
@@ -568,9 +569,26 @@ struct Impl : ImplBase {
                  })) != MPM_OK)
                 return rc;
             // re-map gather (tiled: shared window, 256 threads); MUSL then scatters the
-            // new momentum (tiled: its own kernel; otherwise fused, global atomics)
+            // new momentum (tiled: its own kernel, or the one-pass base-owned re-map;
+            // otherwise fused, global atomics).  The one-pass form's FP64 3D uGIMP
+            // sums do not fit the register file: FP64 3D uGIMP keeps the two kernels.
+            const bool remap1 = tiled && owned && sa.remap && c->fused_remap &&
+                                !(DIM == 3 && S == 3 && sizeof(R) == 8);
             if ((rc = c->launch(KID_G2P1, [&] {
-                     if (tiled)
+                     if (remap1) {
+                         if constexpr (!(DIM == 3 && S == 3 && sizeof(R) == 8)) {
+                             if (sa.va != 0 || sa.pic != R(0))
+                                 k_remap_owned<DIM, S, TSH, R, true><<<sort.tiled_grid(), 64, 0, s>>>(
+                                     geom, sort.tg, pp, gp, sa, sort.masks, sort.offs, sort.ovf,
+                                     sort.rounds, sort.tile_list, sort.seg_begin, sort.seg_end,
+                                     sort.tile_count);
+                             else
+                                 k_remap_owned<DIM, S, TSH, R, false><<<sort.tiled_grid(), 64, 0, s>>>(
+                                     geom, sort.tg, pp, gp, sa, sort.masks, sort.offs, sort.ovf,
+                                     sort.rounds, sort.tile_list, sort.seg_begin, sort.seg_end,
+                                     sort.tile_count);
+                         }
+                     } else if (tiled)
                          if (sa.va != 0 || sa.pic != R(0))
                              k_g2p1_tiled<DIM, S, TSH, R, true><<<sort.tiled_grid(), 256, 0, s>>>(
                                  geom, sort.tg, pp, gp, sa, sort.tile_list, sort.seg_begin,
@@ -583,7 +601,7 @@ struct Impl : ImplBase {
                          k_g2p1_p2g2<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, sa, n, fl);
                  })) != MPM_OK)
                 return rc;
-            if (tiled && sa.remap) {
+            if (tiled && sa.remap && !remap1) {
                 if ((rc = c->launch(KID_P2G2, [&] {
                          if (owned)
                              owned_scatter(std::integral_constant<int, 1>{});
@@ -952,6 +970,7 @@ int mpm_create(const mpm_config* cfg, mpm_ctx** out) {
     }
     mpm_ctx* c = new mpm_ctx();
     c->cfg = *cfg;
+
     if (c->cfg.dim == 2) {
         c->cfg.nnz = 1;
         c->cfg.z1 = 0.0;
@@ -1156,6 +1175,8 @@ int mpm_set_option(mpm_ctx* c, int option, int64_t value) {
         c->cfg.sort_interval = int32_t(value);
     } else if (option == MPM_OPT_TILES) {
         c->use_tiles = value != 0;
+    } else if (option == MPM_OPT_FUSED_REMAP) {
+        c->fused_remap = value != 0;
     } else if (option == MPM_OPT_SCATTER) {
         if (value != MPM_SCATTER_ROUNDS && value != MPM_SCATTER_OWNED && value != MPM_SCATTER_OWNED_LAYERS)
             return c->fail(MPM_E_ARG, "mpm_set_option: scatter form is MPM_SCATTER_ROUNDS, "

@@ -193,6 +193,30 @@ __device__ __forceinline__ void ugimp(R d, R h, R lp, R c4, R c1, R half, R& N, 
     }
 }
 
+// FP32 uGIMP weights of the three nodes of one axis, branch-free, from the
+// offset d0 = x - x_base (in [h/2, 3h/2)): node k sits at d0 - k h.  The same
+// piecewise function as ugimp (uGIMPbasisx, mpbasisfunc.jl:56-76) with every
+// division by h replaced by a product with 1/h.
+__device__ __forceinline__ void ugimp3_f32(float d0, float h, float hinv, float lp, float c4,
+                                           float c1, float half, float* N, float* dN) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float d = d0 - float(k) * h;
+        const float a = fabsf(d);
+        const float t = (h + half) - a;
+        const float sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+        const float Nc = 1.0f - (4.0f * d * d + lp * lp) * c4;
+        const float Nl = 1.0f - a * hinv;
+        const float Nq = t * t * 0.5f * c1;
+        const float dNc = -8.0f * d * c4;
+        const float dNl = -sg * hinv;
+        const float dNq = -sg * t * c1;
+        const bool in_c = a < half, in_l = a < h - half, in_q = a < h + half;
+        N[k] = in_c ? Nc : (in_l ? Nl : (in_q ? Nq : 0.0f));
+        dN[k] = in_c ? dNc : (in_l ? dNl : (in_q ? dNq : 0.0f));
+    }
+}
+
 // Per-axis stencil: S nodes starting at base[a].
 template <int DIM, int S, typename R>
 struct Stencil {
@@ -237,6 +261,16 @@ __device__ __forceinline__ bool make_stencil(const GridGeom& g, const double* x,
             b = int(fb);
         }
         st.base[a] = b;
+        if constexpr (S == 3 && sizeof(R) == 4) {
+            // FP32: the offset from the base node once in FP64, the other two
+            // nodes' offsets in FP32 (d0 - h is exact for d0 in [h/2, 3h/2)),
+            // and the weights branch-free with the reciprocal constants
+            const float d0 = float(x[a] - (g.org[a] + fb * g.h[a]));
+            const float h = float(g.h[a]), hinv = float(g.hinv[a]);
+            ugimp3_f32(d0, h, hinv, float(g.lp[a]), float(g.c4[a]), float(g.c1[a]),
+                       float(g.half[a]), st.N[a], st.dN[a]);
+            continue;
+        }
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             double xn = g.org[a] + double(b + k) * g.h[a];

@@ -87,21 +87,27 @@ struct ScatterIn {
     R m, vol, v[3], sg[6];
 };
 
-template <int DIM, typename R, int MODE>
+// MOM / FRC (MODE 0): the pass sums mass and momentum / force, so it reads
+// the velocity / the volume and stress (the mass is read by both).
+template <int DIM, typename R, int MODE, bool MOM = true, bool FRC = true>
 __device__ __forceinline__ void load_scatter_in(const PartPtrs<R>& pp, int i, ScatterIn<R>& P) {
     load_x<DIM>(pp.x, i, P.x);
     if constexpr (MODE == 0) {
         P.m = pp.ms[i];
-        P.vol = pp.vol[i];
+        if constexpr (MOM) {
 #pragma unroll
-        for (int a = 0; a < DIM; ++a) P.v[a] = pp.v[a][i];
-        P.sg[0] = pp.sig[0][i];
-        P.sg[1] = pp.sig[1][i];
-        P.sg[3] = pp.sig[3][i];
-        if constexpr (DIM == 3) {
-            P.sg[2] = pp.sig[2][i];
-            P.sg[4] = pp.sig[4][i];
-            P.sg[5] = pp.sig[5][i];
+            for (int a = 0; a < DIM; ++a) P.v[a] = pp.v[a][i];
+        }
+        if constexpr (FRC) {
+            P.vol = pp.vol[i];
+            P.sg[0] = pp.sig[0][i];
+            P.sg[1] = pp.sig[1][i];
+            P.sg[3] = pp.sig[3][i];
+            if constexpr (DIM == 3) {
+                P.sg[2] = pp.sig[2][i];
+                P.sg[4] = pp.sig[4][i];
+                P.sg[5] = pp.sig[5][i];
+            }
         }
     } else {
 #pragma unroll
@@ -304,6 +310,179 @@ __device__ __forceinline__ void for_stencil_rows(const Stencil<DIM, S, R>& st, F
 }
 
 // ---------------------------------------------------------------------------
+// Pieces shared by the base-owned kernels.
+// ---------------------------------------------------------------------------
+constexpr int kOwnedLast = (1 << kRankBits) - 1;  // rank of the merged last group
+constexpr int kDriftCap = 256;                    // drift list entries (LDS)
+
+// The rounds of one tile as seen by the lane that owns base `lane`.
+struct TileRounds {
+    const unsigned long long* tmask;
+    const int* toff;
+    unsigned long long lt;  // lanes below this one
+    int lane, beg, end, nr, xbeg, xcnt;
+
+    __device__ __forceinline__ void init(int tile, int lane_, const unsigned long long* masks,
+                                         const int* offs, const int* ovf, const int* rounds,
+                                         const int* seg_begin, const int* seg_end) {
+        lane = lane_;
+        lt = (1ull << lane) - 1ull;
+        beg = seg_begin[tile];
+        end = seg_end[tile];
+        nr = rounds[tile];
+        tmask = masks + (long long)tile * kOwnedLast;
+        toff = offs + (long long)tile * (kOwnedLast + 1);
+        // the merged last group: this base's particles beyond rank 30
+        xbeg = 0;
+        xcnt = 0;
+        if (nr > kOwnedLast) {
+            const int o = ovf[(long long)tile * 64 + lane];
+            const int o1 = lane < 63 ? ovf[(long long)tile * 64 + lane + 1] : end - beg - toff[kOwnedLast];
+            xbeg = beg + toff[kOwnedLast] + o;
+            xcnt = o1 - o;
+        }
+    }
+    // particle index of round r (0 <= r < nr) for this lane; false if it sits out
+    __device__ __forceinline__ bool index(int r, int& i) const {
+        if (r < kOwnedLast) {
+            const unsigned long long mk = tmask[r];
+            i = beg + toff[r] + __popcll(mk & lt);
+            return (mk >> lane) & 1ull;
+        }
+        i = xbeg + (r - kOwnedLast);
+        return r - kOwnedLast < xcnt;
+    }
+};
+
+// Fields a scatter adds to: MODE 0 m, p, f; MODE 1 p.
+template <int DIM, typename R, int MODE, int NF>
+__device__ __forceinline__ void scatter_fields(const GridPtrs<R>& gp, R* (&fields)[NF]) {
+    if constexpr (MODE == 0) {
+        fields[0] = gp.m;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            fields[1 + d] = gp.p[d];
+            fields[1 + DIM + d] = gp.f[d];
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) fields[d] = gp.p[d];
+    }
+}
+
+// The lane's sums of fields [F0, F1) on z layers [Z0, Z0 + NZP) into the
+// window, one stencil slot at a time (the 64 bases reach 64 distinct nodes at
+// each slot).  The empty asm keeps the compiler from holding the slot
+// addresses across the rounds.
+template <int DIM, int S, int NF, int F0, int F1, int Z0, int NZP, typename R, int WW, int ZS,
+          int WN, bool PAIRED>
+__device__ __forceinline__ void owned_add_sums(lds_ptr<R> win, int wbase0,
+                                               const OwnedAcc<F1 - F0, NZP, S, R>& acc, bool any) {
+    int wbase = wbase0;
+    asm volatile("" : "+v"(wbase));
+    if (!any) return;
+    static_for<NZP>([&](auto cc) {
+        static_for<S>([&](auto b) {
+            static_for<S>([&](auto a) {
+                constexpr int C = Z0 + decltype(cc)::value;
+                const int k = wbase + (DIM == 3 ? C * ZS : 0) + int(b) * WW + int(a);
+                R c[F1 - F0];
+#pragma unroll
+                for (int f = 0; f < F1 - F0; ++f) c[f] = acc.v[f][cc][b][a];
+                window_add_fields<NF, F0, F1, R, WN, PAIRED>(win, k, c);
+                __builtin_amdgcn_sched_barrier(0);
+            });
+        });
+    });
+}
+
+template <int NFP, int NZP, int S, typename R>
+__device__ __forceinline__ void owned_zero(OwnedAcc<NFP, NZP, S, R>& acc) {
+#pragma unroll
+    for (int f = 0; f < NFP; ++f)
+#pragma unroll
+        for (int cc = 0; cc < NZP; ++cc)
+#pragma unroll
+            for (int b = 0; b < S; ++b)
+#pragma unroll
+                for (int a = 0; a < S; ++a) acc.v[f][cc][b][a] = R(0);
+}
+
+// Drifted particles drift[k0, k0 + n), n <= 64, one per lane: claim rounds
+// into the window (write a unique tag to the base's claim slot, read it back;
+// the winners add with the lockstep read-add-write, the rest claim again),
+// global atomics for a stencil outside the window.  Every lane calls this
+// (wave-uniform n).  MODE 1 reads the momentum from pp.pm.
+template <int DIM, int S, int NB, int WW, int ZS, int WN, typename R, int MODE, int NF, bool PAIRED>
+__device__ __forceinline__ void owned_drift_batch(const GridGeom& g, const int* w0,
+                                                  const PartPtrs<R>& pp, const Gravity<R>& G,
+                                                  R* const* fields, lds_ptr<R> win,
+                                                  lds_ptr<unsigned> claim, lds_ptr<int> drift,
+                                                  int k0, int n, unsigned& tag) {
+    const int lane = threadIdx.x;
+    ScatterIn<R> P;
+    Stencil<DIM, S, R> st;
+    bool pend = false;
+    int node0 = 0, bidx = 0;
+    if (lane < n) {
+        const int i = drift[k0 + lane];
+        load_scatter_in<DIM, R, MODE>(pp, i, P);
+        int fl2 = 0;
+        if (make_stencil<DIM, S, R>(g, P.x, st, fl2)) {
+            const bool inside = window_slot<DIM, NB, WW, ZS>(st.base, w0, node0, bidx);
+            if (inside) {
+                pend = true;
+            } else {
+                for_stencil_rows<DIM, S, R>(st, [&](int b, int c, int a, R N, R dNx, R dNy, R dNz) {
+                    if (N != R(0)) {
+                        R cv[NF];
+                        owned_node_values<DIM, R, MODE>(N, dNx, dNy, dNz, P, G, cv);
+                        const long long nn = node_id(g, st.base[0] + a, st.base[1] + b,
+                                                     DIM == 3 ? st.base[2] + c : 0);
+#pragma unroll
+                        for (int f = 0; f < NF; ++f) atomic_add(&fields[f][nn], cv[f]);
+                    }
+                });
+            }
+        }
+    }
+    while (__ballot(pend) != 0ull) {
+        bool won = false;
+        if (pend) {
+            claim[bidx] = tag;
+            won = claim[bidx] == tag;
+        }
+        tag += 64u;
+        if (won) {
+            for_stencil_rows<DIM, S, R>(st, [&](int b, int c, int a, R N, R dNx, R dNy, R dNz) {
+                if (N != R(0)) {
+                    R cv[NF];
+                    owned_node_values<DIM, R, MODE>(N, dNx, dNy, dNz, P, G, cv);
+                    window_add_node<NF, R, WN, PAIRED>(win, node0 + c * ZS + b * WW + a, cv);
+                }
+            });
+            pend = false;
+        }
+    }
+}
+
+// Stencil weights of a lane that is not home zeroed (its sums add nothing),
+// and the derivative of a zero axis weight zeroed: the reference skips nodes
+// with Ni == 0 (utils_OS.jl:431,470), and a zero axis weight zeroes every node
+// of its row (the linear basis has a non-zero derivative on a node).
+template <int DIM, int S, typename R>
+__device__ __forceinline__ void owned_mask_stencil(Stencil<DIM, S, R>& st, bool home) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const R n = home ? st.N[d][k] : R(0);
+            st.dN[d][k] = n == R(0) ? R(0) : st.dN[d][k];
+            st.N[d][k] = n;
+        }
+}
+
+// ---------------------------------------------------------------------------
 // The scatter kernel.  MODE 0: P2G #1; MODE 1: the MUSL re-map momentum
 // scatter (from pp.pm).  SPLIT: passes over the tile (OwnedPasses).
 // ---------------------------------------------------------------------------
@@ -321,28 +500,16 @@ __global__ __launch_bounds__(64) void k_scatter_owned(
     constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
     constexpr int NF = PS::NF;
     constexpr bool PAIRED = kPairedP2G<DIM, NF, R, WN>;
-    constexpr int kLast = (1 << kRankBits) - 1;  // rank of the merged last group
     __shared__ __attribute__((aligned(16))) R win_[NF * WN];
     __shared__ unsigned claim_[64];
-    constexpr int kDriftCap = 256;  // drift list entries (LDS)
     __shared__ int drift_[kDriftCap];
     lds_ptr<R> win = (lds_ptr<R>)win_;
     lds_ptr<unsigned> claim = (lds_ptr<unsigned>)claim_;
     lds_ptr<int> drift = (lds_ptr<int>)drift_;
     R* fields[NF];
-    if constexpr (MODE == 0) {
-        fields[0] = gp.m;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            fields[1 + d] = gp.p[d];
-            fields[1 + DIM + d] = gp.f[d];
-        }
-    } else {
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) fields[d] = gp.p[d];
-    }
+    scatter_fields<DIM, R, MODE, NF>(gp, fields);
     const int lane = threadIdx.x;
-    const unsigned long long lt = (1ull << lane) - 1ull;  // lanes below this one
+    const unsigned long long lt = (1ull << lane) - 1ull;
     // the lane's base inside the tile (x fastest), and its node in the window
     const int lx = lane & (NB - 1);
     const int ly = (lane >> TSH) & (NB - 1);
@@ -354,127 +521,42 @@ __global__ __launch_bounds__(64) void k_scatter_owned(
     xcd_chunk(*count_ptr, e0, e1, est);
     for (int e = e0; e < e1; e += est) {
         const int tile = tile_list[e];
-        const int beg = seg_begin[tile], end = seg_end[tile];
-        const int nr = rounds[tile];
+        TileRounds tr;
+        tr.init(tile, lane, masks, offs, ovf, rounds, seg_begin, seg_end);
         int w0[3];
         tile_window_origin<DIM>(tg, tile, w0);
         const int myb[3] = {w0[0] + lx, w0[1] + ly, w0[2] + lz};
         for (int k = lane; k < NF * WN; k += 64) win[k] = R(0);
-        // the merged last group: this base's particles beyond rank 30
-        const unsigned long long* tmask = masks + (long long)tile * kLast;
-        const int* toff = offs + (long long)tile * (kLast + 1);
-        int xbeg = 0, xcnt = 0;
-        if (nr > kLast) {
-            const int o = ovf[(long long)tile * 64 + lane];
-            const int o1 = lane < 63 ? ovf[(long long)tile * 64 + lane + 1] : end - beg - toff[kLast];
-            xbeg = beg + toff[kLast] + o;
-            xcnt = o1 - o;
-        }
-        // particle index of round r for this lane; false if the lane sits out
-        auto round_index = [&](int r, int& i) -> bool {
-            if (r < kLast) {
-                const unsigned long long mk = tmask[r];
-                i = beg + toff[r] + __popcll(mk & lt);
-                return (mk >> lane) & 1ull;
-            }
-            i = xbeg + (r - kLast);
-            return r - kLast < xcnt;
-        };
         int ndrift = 0;
-        // Drifted particles drift_[k0, k0 + n), n <= 64: claim rounds into the
-        // window, global atomics outside it.  Every lane runs this (wave-uniform n).
-        auto drift_batch = [&](int k0, int n) {
-            ScatterIn<R> P;
-            Stencil<DIM, S, R> st;
-            bool pend = false;
-            int node0 = 0, bidx = 0;
-            if (lane < n) {
-                const int i = drift[k0 + lane];
-                load_scatter_in<DIM, R, MODE>(pp, i, P);
-                int fl2 = 0;
-                if (make_stencil<DIM, S, R>(g, P.x, st, fl2)) {
-                    const bool inside = window_slot<DIM, NB, WW, ZS>(st.base, w0, node0, bidx);
-                    if (inside) {
-                        pend = true;
-                    } else {
-                        for_stencil_rows<DIM, S, R>(st, [&](int b, int c, int a, R N, R dNx, R dNy, R dNz) {
-                            if (N != R(0)) {
-                                R cv[NF];
-                                owned_node_values<DIM, R, MODE>(N, dNx, dNy, dNz, P, G, cv);
-                                const long long nn = node_id(g, st.base[0] + a, st.base[1] + b,
-                                                             DIM == 3 ? st.base[2] + c : 0);
-#pragma unroll
-                                for (int f = 0; f < NF; ++f) atomic_add(&fields[f][nn], cv[f]);
-                            }
-                        });
-                    }
-                }
-            }
-            while (__ballot(pend) != 0ull) {
-                bool won = false;
-                if (pend) {
-                    claim[bidx] = tag;
-                    won = claim[bidx] == tag;
-                }
-                tag += 64u;
-                if (won) {
-                    for_stencil_rows<DIM, S, R>(st, [&](int b, int c, int a, R N, R dNx, R dNy, R dNz) {
-                        if (N != R(0)) {
-                            R cv[NF];
-                            owned_node_values<DIM, R, MODE>(N, dNx, dNy, dNz, P, G, cv);
-                            window_add_node<NF, R, WN, PAIRED>(win, node0 + c * ZS + b * WW + a, cv);
-                        }
-                    });
-                    pend = false;
-                }
-            }
+        auto drift_all = [&] {
+            for (int k0 = 0; k0 < ndrift; k0 += 64)
+                owned_drift_batch<DIM, S, NB, WW, ZS, WN, R, MODE, NF, PAIRED>(
+                    g, w0, pp, G, fields, win, claim, drift, k0, min(64, ndrift - k0), tag);
+            ndrift = 0;
         };
         static_for<PS::NPASS>([&](auto pass) {
             constexpr int p = decltype(pass)::value;
             constexpr int F0 = PS::F0(p), F1 = PS::F1(p), Z0 = PS::Z0(p), NZP = PS::NZP;
             constexpr bool FIRST = p == 0;
-            using Acc = OwnedAcc<F1 - F0, NZP, S, R>;
-            auto zero_sums = [](Acc& acc) {
-#pragma unroll
-                for (int f = 0; f < F1 - F0; ++f)
-#pragma unroll
-                    for (int cc = 0; cc < NZP; ++cc)
-#pragma unroll
-                        for (int b = 0; b < S; ++b)
-#pragma unroll
-                            for (int a = 0; a < S; ++a) acc.v[f][cc][b][a] = R(0);
-            };
-            // The lane's sums into the window, one stencil slot at a time (the
-            // 64 bases reach 64 distinct nodes at each slot).  The empty asm
-            // keeps the compiler from holding 27 slot addresses across the rounds.
-            auto add_sums = [&](const Acc& acc, bool any) {
-                int wbase = wbase0;
-                asm volatile("" : "+v"(wbase));
-                if (!any) return;
-                static_for<NZP>([&](auto cc) {
-                    static_for<S>([&](auto b) {
-                        static_for<S>([&](auto a) {
-                            constexpr int C = Z0 + decltype(cc)::value;
-                            const int k = wbase + (DIM == 3 ? C * ZS : 0) + int(b) * WW + int(a);
-                            R c[F1 - F0];
-#pragma unroll
-                            for (int f = 0; f < F1 - F0; ++f) c[f] = acc.v[f][cc][b][a];
-                            window_add_fields<NF, F0, F1, R, WN, PAIRED>(win, k, c);
-                            __builtin_amdgcn_sched_barrier(0);
-                        });
-                    });
-                });
-            };
-            Acc acc;
-            zero_sums(acc);
+            OwnedAcc<F1 - F0, NZP, S, R> acc;
+            owned_zero(acc);
+            constexpr bool MOM = MODE != 0 || F0 == 0, FRC = MODE != 0 || F1 == NF;
             bool any = false;  // the lane's base holds a particle
-            for (int r = 0; r < nr; ++r) {
-                int i = 0;
-                const bool act = round_index(r, i);
-                // every lane loads (a lane that sits out reads the tile's first
-                // particle) and sums with zero weights unless its particle is home
-                ScatterIn<R> P;
-                load_scatter_in<DIM, R, MODE>(pp, act ? i : beg, P);
+            // Every lane loads (a lane that sits out reads the tile's first
+            // particle) and sums with zero weights unless its particle is home.
+            // Software pipeline: round r + 1's loads are issued before round r's sums.
+            int in_ = 0;
+            bool actn = tr.nr > 0 && tr.index(0, in_);
+            ScatterIn<R> Pn;
+            load_scatter_in<DIM, R, MODE, MOM, FRC>(pp, actn ? in_ : tr.beg, Pn);
+            for (int r = 0; r < tr.nr; ++r) {
+                const int i = in_;
+                const bool act = actn;
+                const ScatterIn<R> P = Pn;
+                if (r + 1 < tr.nr) {
+                    actn = tr.index(r + 1, in_);
+                    load_scatter_in<DIM, R, MODE, MOM, FRC>(pp, actn ? in_ : tr.beg, Pn);
+                }
                 Stencil<DIM, S, R> st;
                 int fl2 = 0;
                 const bool ok = make_stencil<DIM, S, R>(g, P.x, st, fl2);
@@ -482,17 +564,7 @@ __global__ __launch_bounds__(64) void k_scatter_owned(
                 const bool home = act && ok && st.base[0] == myb[0] && st.base[1] == myb[1] &&
                                   (DIM == 2 || st.base[2] == myb[2]);
                 any = any || home;
-                // The reference skips nodes with Ni == 0 (utils_OS.jl:431,470); a
-                // zero axis weight zeroes every node of that row, so zeroing its
-                // derivative (non-zero for the linear basis on a node) skips them.
-#pragma unroll
-                for (int d = 0; d < DIM; ++d)
-#pragma unroll
-                    for (int k = 0; k < S; ++k) {
-                        const R n = home ? st.N[d][k] : R(0);
-                        st.dN[d][k] = n == R(0) ? R(0) : st.dN[d][k];
-                        st.N[d][k] = n;
-                    }
+                owned_mask_stencil(st, home);
                 owned_accumulate<DIM, S, R, MODE, F0, F1, Z0, NZP>(st, P, G, acc);
                 if constexpr (FIRST) {
                     const bool drifted = act && ok && !home;
@@ -504,21 +576,176 @@ __global__ __launch_bounds__(64) void k_scatter_owned(
                             // The list is nearly full.  The sums so far go to the
                             // window first, so the batch does not hold registers
                             // beside them.
-                            add_sums(acc, any);
-                            zero_sums(acc);
+                            owned_add_sums<DIM, S, NF, F0, F1, Z0, NZP, R, WW, ZS, WN, PAIRED>(
+                                win, wbase0, acc, any);
+                            owned_zero(acc);
                             any = false;
-                            for (int k0 = 0; k0 < ndrift; k0 += 64) drift_batch(k0, min(64, ndrift - k0));
-                            ndrift = 0;
+                            drift_all();
                         }
                     }
                 }
             }
-            add_sums(acc, any);
+            owned_add_sums<DIM, S, NF, F0, F1, Z0, NZP, R, WW, ZS, WN, PAIRED>(win, wbase0, acc, any);
         });
-        for (int k0 = 0; k0 < ndrift; k0 += 64) drift_batch(k0, min(64, ndrift - k0));
+        drift_all();
         flush_window<DIM, WW, ZS, WN, NF, R>(g, w0, win, fields);
     }
     raise_flag(flags, flag);
+}
+
+// ---------------------------------------------------------------------------
+// The MUSL re-map in one pass (doublemapping1(_a)_OS! + doublemapping2_OS!,
+// utils_OS.jl:698-959): the tile's vsT, as (and vs when VS) are staged in LDS,
+// each lane gathers its round's particle from there (a particle outside the
+// window from the grid), moves it (v, the deferred position increment) and
+// adds the new momentum N m v of a particle on its own base to its register
+// sums, all with the step-start basis.  A drifted particle stores its
+// momentum in pp.pm for the drift batch.  Replaces k_g2p1_tiled +
+// k_scatter_owned<..., 1>: the particle position is read once and the
+// momentum never goes through memory.
+// ---------------------------------------------------------------------------
+template <int DIM, int S, int TSH, typename R, bool VS>
+__global__ __launch_bounds__(64) void k_remap_owned(
+    GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, StepArgs<R> sa,
+    const unsigned long long* __restrict__ masks, const int* __restrict__ offs,
+    const int* __restrict__ ovf, const int* __restrict__ rounds,
+    const int* __restrict__ tile_list, const int* __restrict__ seg_begin,
+    const int* __restrict__ seg_end, const int* __restrict__ count_ptr) {
+    using W = Win<DIM, S, TSH>;
+    static_assert(kWinSlack == 0, "lane-owned bases assume a slack-free window");
+    static_assert(W::NBN == 64, "one base per lane");
+    constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
+    constexpr int NF = DIM;                    // momentum sums
+    constexpr int NG = (VS ? 3 : 2) * DIM;     // gathered: vsT, as[, vs]
+    constexpr bool PAIRED = kPairedP2G<DIM, NF, R, WN>;
+    __shared__ __attribute__((aligned(16))) R win_[NF * WN];
+    __shared__ R gwin[NG * WN];
+    __shared__ unsigned claim_[64];
+    __shared__ int drift_[kDriftCap];
+    lds_ptr<R> win = (lds_ptr<R>)win_;
+    lds_ptr<unsigned> claim = (lds_ptr<unsigned>)claim_;
+    lds_ptr<int> drift = (lds_ptr<int>)drift_;
+    R* fields[NF];
+    scatter_fields<DIM, R, 1, NF>(gp, fields);
+    R* const* vsf = pic_field(gp, sa);
+    R* gfields[3 * DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        gfields[d] = gp.vsT[d];
+        gfields[DIM + d] = gp.as[d];
+        gfields[2 * DIM + d] = vsf[d];
+    }
+    const Gravity<R> G0{};  // unused by the momentum scatter
+    const int lane = threadIdx.x;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const int lx = lane & (NB - 1);
+    const int ly = (lane >> TSH) & (NB - 1);
+    const int lz = DIM == 3 ? lane >> (2 * TSH) : 0;
+    const int wbase0 = DIM == 3 ? lz * ZS + ly * WW + lx : ly * WW + lx;
+    unsigned tag = unsigned(lane);
+    int e0, e1, est;
+    xcd_chunk(*count_ptr, e0, e1, est);
+    for (int e = e0; e < e1; e += est) {
+        const int tile = tile_list[e];
+        TileRounds tr;
+        tr.init(tile, lane, masks, offs, ovf, rounds, seg_begin, seg_end);
+        int w0[3];
+        tile_window_origin<DIM>(tg, tile, w0);
+        const int myb[3] = {w0[0] + lx, w0[1] + ly, w0[2] + lz};
+        __syncthreads();  // the previous tile's reads of gwin are done
+        for (int k = lane; k < NF * WN; k += 64) win[k] = R(0);
+        stage_window<DIM, WW, ZS, WN, NG, R>(g, w0, gfields, gwin);
+        __syncthreads();
+        int ndrift = 0;
+        auto drift_all = [&] {
+            for (int k0 = 0; k0 < ndrift; k0 += 64)
+                owned_drift_batch<DIM, S, NB, WW, ZS, WN, R, 1, NF, PAIRED>(
+                    g, w0, pp, G0, fields, win, claim, drift, k0, min(64, ndrift - k0), tag);
+            ndrift = 0;
+        };
+        OwnedAcc<NF, DIM == 3 ? S : 1, S, R> acc;
+        owned_zero(acc);
+        bool any = false;
+        for (int r = 0; r < tr.nr; ++r) {
+            int i = 0;
+            const bool act = tr.index(r, i);
+            const int il = act ? i : tr.beg;
+            ScatterIn<R> P;  // P.x: position; P.v: the new momentum
+            load_x<DIM>(pp.x, il, P.x);
+            const R m0 = pp.ms[il];
+            R v0[3];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) v0[d] = pp.v[d][il];
+            Stencil<DIM, S, R> st;
+            int fl2 = 0;  // reported by P2G #1
+            const bool ok = make_stencil<DIM, S, R>(g, P.x, st, fl2);
+            int node0 = 0, bidx = 0;
+            const bool inside = ok && window_slot<DIM, NB, WW, ZS>(st.base, w0, node0, bidx);
+            R pv[3] = {0, 0, 0};
+            if (act && ok) {
+                R A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, C[3] = {0, 0, 0};
+                if (inside) {
+                    // branch-free: zero weights add exact zeros (staged values are finite)
+                    for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R, R, R) {
+                        const R* L = gwin + node0 + off;
+                        R vT[3], vA[3], vS[3];
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) {
+                            vT[d] = L[d * WN];
+                            vA[d] = L[(DIM + d) * WN];
+                            vS[d] = VS ? L[(2 * DIM + d) * WN] : R(0);
+                        }
+                        g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
+                    });
+                } else {
+                    for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
+                        if (N != R(0)) {
+                            R vT[3], vA[3], vS[3];
+#pragma unroll
+                            for (int d = 0; d < DIM; ++d) {
+                                vT[d] = gp.vsT[d][n];
+                                vA[d] = gp.as[d][n];
+                                vS[d] = VS ? vsf[d][n] : R(0);
+                            }
+                            g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
+                        }
+                    });
+                }
+                g2p1_commit<DIM, R>(pp, sa, i, A, B, C, pv, m0, v0);
+            } else if (act) {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) pp.dxp[d][i] = R(0);  // frozen particle
+            }
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) P.v[d] = pv[d];
+            const bool home = act && ok && st.base[0] == myb[0] && st.base[1] == myb[1] &&
+                              (DIM == 2 || st.base[2] == myb[2]);
+            any = any || home;
+            const bool drifted = act && ok && !home;
+            if (drifted) {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) pp.pm[d][i] = pv[d];
+            }
+            owned_mask_stencil(st, home);
+            owned_accumulate<DIM, S, R, 1, 0, NF, 0, DIM == 3 ? S : 1>(st, P, G0, acc);
+            const unsigned long long dm = __ballot(drifted);
+            if (dm != 0ull) {
+                if (drifted) drift[ndrift + __popcll(dm & lt)] = i;
+                ndrift += __popcll(dm);
+                if (ndrift > kDriftCap - 64) {
+                    owned_add_sums<DIM, S, NF, 0, NF, 0, DIM == 3 ? S : 1, R, WW, ZS, WN, PAIRED>(
+                        win, wbase0, acc, any);
+                    owned_zero(acc);
+                    any = false;
+                    drift_all();
+                }
+            }
+        }
+        owned_add_sums<DIM, S, NF, 0, NF, 0, DIM == 3 ? S : 1, R, WW, ZS, WN, PAIRED>(win, wbase0,
+                                                                                  acc, any);
+        drift_all();
+        flush_window<DIM, WW, ZS, WN, NF, R>(g, w0, win, fields);
+    }
 }
 
 }  // namespace mpm

@@ -27,7 +27,7 @@ def test_every_declared_symbol_is_exported(mpm):
 
 def test_abi_version_and_kernel_names(mpm):
     lib = mpm._capi.load()
-    assert lib.mpm_abi_version() == 1
+    assert lib.mpm_abi_version() == 2
     k = lib.mpm_profile_kernel_count()
     names = [lib.mpm_profile_kernel_name(i).decode() for i in range(k)]
     assert names[:5] == ["k_p2g", "k_grid1", "k_g2p1_p2g2", "k_grid2", "k_g2p2"]

@@ -1,10 +1,18 @@
 """Full-run checks on the GPU.
 
 * The plane-strain column collapse at full resolution (12,800 particles,
-  h = 2.5 mm, dt = 2e-5 s, 50,000 steps = 1 s) in FP64 against the
-  experimental free surface the reference ships, and FP32 (FP64 positions and
-  det F) against FP64: runout within 1 %, kinetic-energy history and deposit
-  profile close.
+  h = 2.5 mm, dt = 2e-5 s, 50,000 steps = 1 s): FP64 against the experimental
+  free surface the reference ships, and FP32 (FP64 positions and det F)
+  against FP64 within 0.5 % on runout, kinetic-energy history and deposit
+  profile.  The collapse is chaotic: an FP64 run whose initial positions are
+  moved by 1e-9 h (2.5e-12 m) ends with its 99.9 % runout quantile 0.16-0.26 %
+  away, while moves of 1e-12 h leave it unchanged; FP32 rounding (~6e-8
+  relative per operation) is far above that threshold, so every FP32 run is
+  one draw from the same spread.  Runout is therefore compared as the mean over
+  four realizations of each precision (FP64: unperturbed and three 1e-9 h
+  perturbations; FP32: the same four starts, plain and tile kernels); the
+  kinetic-energy history and the deposit profile, which the spread barely
+  moves, run by run.
 * The 3D column at full size (2,048,000 particles): size-independent
   properties (mass conserved exactly, every particle finite and inside the
   grid, FP32 vs FP64 agreement over a short horizon).
@@ -16,17 +24,22 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+H = 0.0025
 
 
 def runout(x, q=0.999):
-    """Robust runout: the q-quantile of the final particle x positions."""
+    """Runout: the q-quantile of the final particle x positions."""
     return float(np.quantile(x[:, 0], q))
 
 
-def _run_2d(mpm, eps, chunk=1000, total=50_000):
+def _run_2d(mpm, eps, sort_interval=0, perturb=0.0, seed=0, chunk=1000, total=50_000):
     args, grid, mp, attr, bc = mpm.scenes.column_collapse_2d(eps=eps)
-    print(f"case: collapse2d eps={eps} np={mp.np} steps={total}", flush=True)
-    model = mpm.HIPModel(grid, mp, attr, bc, sort_interval=0)
+    if perturb:
+        rng = np.random.default_rng(seed)
+        mp.ξ[:] = mp.ξ + perturb * H * rng.standard_normal(mp.ξ.shape)
+    print(f"case: collapse2d eps={eps} np={mp.np} steps={total} sort={sort_interval} "
+          f"perturb={perturb} seed={seed}", flush=True)
+    model = mpm.HIPModel(grid, mp, attr, bc, sort_interval=sort_interval)
     ke = []
     try:
         sargs = model.step_args(args)
@@ -39,37 +52,49 @@ def _run_2d(mpm, eps, chunk=1000, total=50_000):
         model.sync()
     finally:
         model.close()
-    return s, np.array(ke)
+    return s["xi"], np.array(ke)
+
+
+def _profile(x):
+    """Deposit profile: the highest particle in each 5 mm bin of x."""
+    bins = np.arange(0.0, 0.5, 0.005)
+    return np.array([x[(x[:, 0] >= b) & (x[:, 0] < b + 0.005), 1].max(initial=0) for b in bins])
 
 
 def test_collapse2d_full_fp64_vs_experiment_and_fp32(mpm):
-    s64, ke64 = _run_2d(mpm, "FP64")
-    s32, ke32 = _run_2d(mpm, "FP32")
-    x64, x32 = s64["xi"], s32["xi"]
+    starts = [(0.0, 0), (1e-9, 1), (1e-9, 2), (1e-9, 3)]
+    runs64 = [_run_2d(mpm, "FP64", perturb=p, seed=s) for p, s in starts]
+    # FP32: plain kernels for two starts, tile kernels (sorted every 10 steps) for two
+    runs32 = [_run_2d(mpm, "FP32", sort_interval=si, perturb=p, seed=s)
+              for (p, s), si in zip(starts, (0, 10, 0, 10))]
+    x64, ke64 = runs64[0]
     surf = np.loadtxt(os.path.join(GOLDEN, "collapse2d_surface.csv"), delimiter=",")
     err = []
     for xe, ye in surf:
         if xe > 0.40:
             continue
-        sel = np.abs(x64[:, 0] - xe) < 0.0025
+        sel = np.abs(x64[:, 0] - xe) < H
         err.append(x64[sel, 1].max() - ye)
     err = np.array(err)
     print("surface error rms", np.sqrt(np.mean(err ** 2)), "max", np.abs(err).max())
     assert np.sqrt(np.mean(err ** 2)) < 0.004 and np.abs(err).max() < 0.008
-    r64, r32 = runout(x64), runout(x32)
-    print("runout fp64", r64, "fp32", r32, "rel", abs(r32 - r64) / r64)
-    # the FP32 run's atomic summation order differs from run to run, and this
-    # 50,000-step granular flow amplifies it: two runs of one build measured
-    # 0.22 % and 0.68 % for this 99.9 % quantile of x (a few particle spacings)
-    assert abs(r32 - r64) / r64 < 0.01
-    # kinetic-energy history (sampled every 1000 steps)
-    assert np.max(np.abs(ke32 - ke64)) < 0.05 * np.max(ke64)
-    # deposit profiles: max height in 5 mm bins
-    bins = np.arange(0.0, 0.5, 0.005)
-    h64 = np.array([x64[(x64[:, 0] >= b) & (x64[:, 0] < b + 0.005), 1].max(initial=0) for b in bins])
-    h32 = np.array([x32[(x32[:, 0] >= b) & (x32[:, 0] < b + 0.005), 1].max(initial=0) for b in bins])
-    print("deposit profile max diff", np.abs(h64 - h32).max())
-    assert np.abs(h64 - h32).max() < 0.005
+    r64 = np.array([runout(x) for x, _ in runs64])
+    r32 = np.array([runout(x) for x, _ in runs32])
+    rel = abs(r32.mean() - r64.mean()) / r64.mean()
+    print("runout FP64 realizations", r64, "FP32 realizations", r32)
+    print(f"runout mean FP64 {r64.mean():.5f} m, FP32 {r32.mean():.5f} m, rel {rel:.3%}; "
+          f"FP64 spread {np.ptp(r64) / r64.mean():.3%}")
+    assert rel < 0.005
+    h64 = _profile(x64)
+    for x32, ke32 in runs32:
+        # kinetic-energy history (sampled every 1000 steps) against the FP64 run
+        dke = np.max(np.abs(ke32 - ke64)) / np.max(ke64)
+        h32 = _profile(x32)
+        occ = (h32 > 0) | (h64 > 0)
+        dh = np.abs(h32 - h64)[occ].mean() / 0.1  # of the initial column height
+        print(f"FP32 run: kinetic energy max diff {dke:.3%} of the peak, deposit mean |diff| "
+              f"{dh:.3%} of the column height")
+        assert dke < 0.005 and dh < 0.005
 
 
 def test_column3d_full_size_invariants(mpm):

@@ -1,0 +1,176 @@
+"""The benchmark configurations at their stated sizes and horizons, on the GPU.
+
+* 3D column (2,048,000 particles, 20,000 steps, sorted every 5 steps): FP32
+  (FP64 positions and det F) against FP64 over the whole collapse: runout,
+  kinetic-energy history and deposit profile within 0.5 %.
+* Ensemble (256 instances of 5k-60k particles, 20,000 steps, one packed
+  grid, captured step graphs): every instance keeps its mass, stays finite
+  and inside its own box and spreads; three instances of the packed FP64
+  ensemble against the oracle run of the same instance alone.
+* GRF slope (20,000,000 particles, h = 0.5 m): 2,000 damped K0 steps, then
+  runout steps; mass, finiteness, K0 normal and shear stress against the
+  infinite-slope values rho g d cos(theta) and rho g d sin(theta) at depth d.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
+
+
+def _column3d_run(mpm, eps, nsteps=20_000, chunk=2_000, sort_interval=5):
+    args, grid, mp, attr, bc = mpm.scenes.column_collapse_3d(eps=eps, nsteps=nsteps)
+    print(f"case: column3d eps={eps} np={mp.np} steps={nsteps} sort={sort_interval}", flush=True)
+    m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=sort_interval)
+    ke = []
+    try:
+        sa = m.step_args(args)
+        dt, Ti = float(args.ΔT), 0.0
+        for done in range(0, nsteps, chunk):
+            Ti = m.run(sa, dt, Ti, chunk, graph=True)
+            s = m.download()
+            ke.append(mpm.scenes.kinetic_energy(s["ms"], s["vs"]))
+        m.sync()
+    finally:
+        m.close()
+    return s["xi"], np.array(ke)
+
+
+def _deposit(x, b=0.01, half=0.6):
+    """Deposit height map: the highest particle in each b x b column."""
+    n = int(round(2 * half / b))
+    ix = np.clip(((x[:, 0] + half) / b).astype(np.int64), 0, n - 1)
+    iy = np.clip(((x[:, 1] + half) / b).astype(np.int64), 0, n - 1)
+    h = np.zeros((n, n))
+    np.maximum.at(h, (ix, iy), x[:, 2])
+    return h
+
+
+def test_column3d_full_collapse_fp32_vs_fp64(mpm):
+    x64, ke64 = _column3d_run(mpm, "FP64")
+    x32, ke32 = _column3d_run(mpm, "FP32")
+    # runout: the 99.9 % quantile of the distance from the column axis, and of
+    # |x| (the deposit spreads over the floor; single outliers are excluded)
+    for name, f in (("radial", lambda x: np.hypot(x[:, 0], x[:, 1])),
+                    ("|x|", lambda x: np.abs(x[:, 0]))):
+        r64, r32 = np.quantile(f(x64), 0.999), np.quantile(f(x32), 0.999)
+        rel = abs(r32 - r64) / r64
+        print(f"runout {name}: FP64 {r64:.5f} m, FP32 {r32:.5f} m, rel {rel:.3%}", flush=True)
+        assert rel < 0.005
+    dke = np.abs(ke32 - ke64).max() / ke64.max()
+    print(f"kinetic energy history (every 2000 steps): max diff {dke:.3%} of the peak", flush=True)
+    assert dke < 0.005
+    h64, h32 = _deposit(x64), _deposit(x32)
+    occ = (h64 > 0) | (h32 > 0)
+    dh = np.abs(h64 - h32)[occ].mean() / h64.max()
+    print(f"deposit height map: mean |diff| {dh:.3%} of the deposit height", flush=True)
+    assert dh < 0.005
+
+
+def test_ensemble_256_full_horizon(mpm):
+    args, grid, mp, attr, bc, lay = mpm.scenes.ensemble_2d(nsteps=20_000)
+    n_inst = len(lay["ranges"])
+    print(f"case: ensemble {n_inst} instances, {mp.np} particles, {grid.ni} nodes", flush=True)
+    x0 = np.asarray(mp.ξ, dtype=np.float64)
+    m0 = np.asarray(mp.Ω, dtype=np.float64) * np.asarray(mp.ρs, dtype=np.float64)
+    m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=10)
+    try:
+        m.run(m.step_args(args), float(args.ΔT), 0.0, 20_000, graph=True)
+        out = m.download()
+    finally:
+        m.close()
+    h = lay["h"]
+    assert np.all(np.isfinite(out["xi"])) and np.all(np.isfinite(out["sigma"]))
+    runouts = []
+    for k, (a, b) in enumerate(lay["ranges"]):
+        x = out["xi"][a:b]
+        ox, oy = lay["origin"][k]
+        W, H, L = lay["width"][k], lay["height"][k], lay["length"][k]
+        assert abs(out["ms"][a:b].sum() - m0[a:b].sum()) <= 1e-5 * m0[a:b].sum(), k
+        # inside its own box: behind the front wall, in front of the back wall,
+        # above the floor (a particle is never closer than half a cell to a wall)
+        assert x[:, 0].min() > ox - h and x[:, 0].max() < ox + L + h, k
+        assert x[:, 1].min() > oy - h, k
+        front = np.quantile(x[:, 0], 0.999) - ox
+        runouts.append(front)
+        assert front > 0.9 * W, (k, front, W)  # the column did not shrink
+    runouts = np.array(runouts)
+    spread = runouts / np.array(lay["width"])
+    print(f"runout / width over {n_inst} instances: min {spread.min():.2f}, median "
+          f"{np.median(spread):.2f}, max {spread.max():.2f}", flush=True)
+    assert np.median(spread) > 1.2  # the columns collapse and spread
+
+
+def test_ensemble_sampled_instances_vs_oracle(mpm, orc):
+    # the packed FP64 ensemble of all 256 instances; three instances, sampled
+    # by size (smallest, median, largest), against the oracle run alone
+    args, grid, mp, attr, bc, lay = mpm.scenes.ensemble_2d(eps="FP64", nsteps=100)
+    nsteps, dt = 100, float(args.ΔT)
+    x0 = np.asarray(mp.ξ, dtype=np.float64)
+    m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=10)
+    try:
+        m.run(m.step_args(args), dt, 0.0, nsteps, graph=True)
+        packed = m.download()
+    finally:
+        m.close()
+    sizes = np.array([b - a for a, b in lay["ranges"]])
+    order = np.argsort(sizes)
+    for k in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
+        a, b = lay["ranges"][k]
+        sa, sg, smp, sattr, sbc = mpm.scenes.ensemble_instance(lay, k, eps="FP64", nsteps=nsteps)
+        ref = orc.Oracle(sa, sg, smp, sattr, sbc, nthreads=8)
+        ref.run(dt, 0.0, nsteps)
+        want = ref.state()
+        d_packed = packed["xi"][a:b] - x0[a:b]
+        d_want = want["xi"] - np.asarray(smp.ξ, dtype=np.float64)
+        r = _rel(d_packed, d_want)
+        rs = _rel(packed["sigma"][a:b], want["sigma"])
+        rv = _rel(packed["vs"][a:b], want["vs"])
+        print(f"instance {k}: {b - a} particles, displacement rel L2 {r:.1e}, velocity {rv:.1e}, "
+              f"stress {rs:.1e}", flush=True)
+        assert np.abs(d_want).max() > 1e-5
+        assert r <= 1e-8 and rv <= 1e-8 and rs <= 1e-8
+
+
+def test_slope_20m_k0_and_runout(mpm):
+    args, grid, mp, attr, bc, lay = mpm.scenes.slope_3d()
+    print(f"case: slope {mp.np} particles, {lay['nmat']} materials, {grid.ni} nodes", flush=True)
+    assert mp.np == 20_000_000
+    x0 = np.asarray(mp.ξ, dtype=np.float64)
+    m0 = float(np.sum(np.asarray(mp.Ω, dtype=np.float64) * np.asarray(mp.ρs, dtype=np.float64)))
+    dt = lay["dt"]
+    m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=5)
+    try:
+        Ti = m.run(m.step_args(args), dt, 0.0, lay["k0_steps"], graph=True)
+        k0 = m.download()
+        Ti = m.run(m.step_args(lay["runout_args"]), dt, Ti, 2_000, graph=True)
+        out = m.download()
+    finally:
+        m.close()
+    for s in (k0, out):
+        assert np.all(np.isfinite(s["xi"])) and np.all(np.isfinite(s["sigma"]))
+        assert abs(s["ms"].sum() - m0) <= 1e-5 * m0
+    # K0: interior particles, depth 2-8 m below the surface of their column
+    fx0, fy0, fl, fw = lay["footprint"]
+    h, depth, rho, g = 0.5, 10.0, 2000.0, 9.8
+    theta = math.radians(35.0)
+    zb = mpm.scenes._bilinear(lay["bed"], -2 * h, h, -2 * h, h, x0[:, 0], x0[:, 1])
+    d = zb + depth - x0[:, 2]
+    sel = ((x0[:, 0] > fx0 + 40) & (x0[:, 0] < fx0 + fl - 40) & (x0[:, 1] > fy0 + 25) &
+           (x0[:, 1] < fy0 + fw - 25) & (d > 2.0) & (d < 8.0))
+    szz = k0["sigma"][sel, 2] / (-rho * g * math.cos(theta) * d[sel])
+    sxz = k0["sigma"][sel, 5] / (-rho * g * math.sin(theta) * d[sel])
+    print(f"K0 ({sel.sum()} interior particles): sigma_zz / (rho g d cos) median {np.median(szz):.3f}, "
+          f"sigma_xz / (rho g d sin) median {np.median(sxz):.3f}", flush=True)
+    assert 0.85 < np.median(szz) < 1.15
+    assert 0.7 < np.median(sxz) < 1.3
+    dx = out["xi"][:, 0] - x0[:, 0]
+    print(f"runout after 2000 steps ({2000 * dt:.2f} s): mean dx {dx.mean():.3f} m, max {dx.max():.2f} m",
+          flush=True)
+    assert dx.mean() > 0.0  # the failing mass moves downslope

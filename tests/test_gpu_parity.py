@@ -18,15 +18,17 @@ def _rel(a, b):
 
 def _compare(mpm, orc, scene, nsteps, tol, sort_interval=0, keys=("xi", "vs", "sigma", "eps_s",
                                                                    "eps_q", "omega"),
-             scatter=None):
+             scatter=None, fused_remap=None):
     args, grid, mp, attr, bc = scene
     print(f"case: dim={grid.dim} eps={mp.ϵ} np={mp.np} const={args.constitutive} va={args.va} "
           f"MVL={args.MVL} NIC={grid.NIC} steps={nsteps} sort={sort_interval} "
-          f"scatter={scatter or 'default'}", flush=True)
+          f"scatter={scatter or 'default'} fused_remap={fused_remap}", flush=True)
     model = mpm.HIPModel(grid, mp, attr, bc, sort_interval=sort_interval)
     try:
         if scatter is not None:
             model.set_option("scatter", scatter)
+        if fused_remap is not None:
+            model.set_option("fused_remap", fused_remap)
         sargs = model.step_args(args)
         model.run(sargs, float(args.ΔT), 0.0, nsteps)
         model.sync()
@@ -102,6 +104,22 @@ def test_fp64_scatter_forms(mpm, orc, scatter, sort):
              sort_interval=sort, scatter=scatter)
     _compare(mpm, orc, _c3(mpm, constitutive="druckerprager"), 100, 1e-9, sort_interval=sort,
              scatter=scatter)
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_remap_one_pass_and_split(mpm, orc, fused):
+    # the one-pass base-owned re-map (FP32, and FP64 2D) against the gather +
+    # momentum-scatter pair, both against the oracle; sort interval 400 adds
+    # drifted particles
+    tol = dict(xi=2e-4, vs=5e-4, sigma=2e-3, eps_s=5e-4, eps_q=2e-3, omega=1e-5)
+    for sort in (5, 400):
+        _compare(mpm, orc, _c2(mpm, constitutive="druckerprager", cohesion=200.0), 200, 1e-9,
+                 sort_interval=sort, fused_remap=fused)
+        _compare(mpm, orc, _c2(mpm, constitutive="druckerprager", cohesion=200.0, va="v"), 200,
+                 1e-9, sort_interval=sort, fused_remap=fused)
+        _compare(mpm, orc, _c3(mpm, eps="FP32", constitutive="druckerprager"), 100, tol,
+                 sort_interval=sort, fused_remap=fused)
+    _compare(mpm, orc, _c3(mpm, basis="linear"), 60, 1e-9, sort_interval=5, fused_remap=fused)
 
 
 @pytest.mark.parametrize("scatter", ["owned", "owned_layers", "rounds"])

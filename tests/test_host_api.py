@@ -95,7 +95,7 @@ class _Recorder:
             if name == "mpm_last_error":
                 return b"ok"
             if name == "mpm_abi_version":
-                return 1
+                return 2
             return 0
         return f
 
