@@ -180,9 +180,13 @@ int mpm_run_graph(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int6
  *                         re-map gathers, moves the particles and sums the new
  *                         momentum in one kernel (not FP64 3D uGIMP, whose sums
  *                         exceed the register file); 0: a gather kernel, then
- *                         the momentum scatter. */
+ *                         the momentum scatter.
+ *  MPM_OPT_TILE_PARTIALS: 1 (default): the tile scatters store each tile's
+ *                         window sums and the grid solves add the windows that
+ *                         hold a node, in a fixed order (no flush atomics);
+ *                         0: the windows are flushed with global atomics. */
 enum { MPM_OPT_SORT_INTERVAL = 0, MPM_OPT_TILES = 1, MPM_OPT_PROFILE_MASK = 2, MPM_OPT_SCATTER = 3,
-       MPM_OPT_FUSED_REMAP = 4 };
+       MPM_OPT_FUSED_REMAP = 4, MPM_OPT_TILE_PARTIALS = 5 };
 enum { MPM_SCATTER_ROUNDS = 0, MPM_SCATTER_OWNED = 1, MPM_SCATTER_OWNED_LAYERS = 2 };
 int mpm_set_option(mpm_ctx *ctx, int option, int64_t value);
 
@@ -233,11 +237,18 @@ int mpm_tpeak(int64_t nbytes, int reps, double *gbps);
 /* ---------------------------------------------------------------------------
  * Learned return map (no reference counterpart: the reference has no
  * surrogate).  A 16-128-128-128-7 SiLU MLP run on the matrix cores in BF16
- * with FP32 accumulation.  Inputs per sample (16, raw units): trial stress
- * [xx,yy,zz,xy,yz,xz] after the elastic predictor, strain increment (same
- * order, engineering shears), accumulated plastic strain eps_q, friction angle
- * (rad), cohesion in effect (Pa), dilation angle (rad).  Outputs (7): stress
- * correction (returned - trial, 6) and the plastic strain increment of eps_q.
+ * with FP32 accumulation.  Inputs per sample (16): the trial stress as its
+ * principal values p1 >= p2 >= p3 over the scale S = max|p| + c cos(phi) (3)
+ * and its principal frame as a unit quaternion (x, y, z; w >= 0) (3), the
+ * strain increment [xx,yy,zz,xy,yz,xz] (engineering shears) times 2 G / S (6),
+ * the accumulated plastic strain eps_q, the friction angle (rad), c / S and the
+ * dilation angle (rad), with c, phi, psi the strength in effect.  Outputs (7):
+ * scores of the four branches of the principal-space Mohr-Coulomb return
+ * (main plane, edge s1 = s2, edge s2 = s3, apex) and three unused outputs
+ * (trained to zero).  In a step (mpm_set_surrogate) a yielding particle takes
+ * the highest-scoring branch's closed form when that branch's conditions and
+ * the yield residual check hold, else the exact return; the return-mapping
+ * log (mpm_return_log) records these inputs with the exact branch one-hot.
  * Parameters: one FP32 vector of mpm_mlp_param_count() values, W1[128][16] b1
  * W2[128][128] b2 W3[128][128] b3 W4[7][128] b4 followed by x_mean[16]
  * x_istd[16] y_scale[7] (inputs are standardised, outputs are scaled).
@@ -268,18 +279,20 @@ int mpm_mlp_bench(mpm_mlp *m, int64_t n, int reps, double *ms_per_launch);
 
 /* Attach a trained network as the 3D Mohr-Coulomb return of this context (FP32
  * mode, 3D, no MVL; NULL detaches).  Each plastic step then stores the trial
- * stress, evaluates the network for every particle, and keeps its correction
- * where the corrected stress satisfies the yield function to
- * |f| <= tol (2 c cos(phi) + |s1 + s3| sin(phi)) with a non-negative eps_q
- * increment; every other yielding particle takes the exact return.  The
- * network must outlive the attachment.  Resets the counters below. */
+ * stress with its principal values and frame (G2P #2), evaluates the network
+ * for every particle, and returns a yielding particle on the branch the
+ * network scores highest when that branch's conditions hold and the returned
+ * stress satisfies the yield function to |f| <= tol (2 c cos(phi) +
+ * |s1 + s3| sin(phi)); every other yielding particle takes the exact return.
+ * The network must outlive the attachment.  Resets the counters below. */
 int mpm_set_surrogate(mpm_ctx *ctx, mpm_mlp *m, double tol);
 /* Yielding particle-steps and fallbacks to the exact return since attaching. */
 int mpm_surrogate_stats(mpm_ctx *ctx, int64_t *plastic, int64_t *fallbacks);
 /* Log return-mapping samples of the exact 3D Mohr-Coulomb return: every
  * stride-th particle (in the current particle order) that yields, up to
- * `capacity` samples (0 disables and frees); samples are the network's inputs
- * and outputs in raw units.  Resets the log. */
+ * `capacity` samples (0 disables and frees); a sample is the network's 16
+ * inputs and the exact return's branch as a one-hot over the first four
+ * outputs (the last three zero).  Resets the log. */
 int mpm_return_log(mpm_ctx *ctx, int64_t capacity, int64_t stride);
 /* Copy up to max_rows logged samples (x: rows x 16, y: rows x 7, row-major). */
 int mpm_return_log_read(mpm_ctx *ctx, float *x, float *y, int64_t max_rows, int64_t *rows);

@@ -69,6 +69,7 @@ struct mpm_ctx {
     bool use_tiles = true;  // LDS-binned scatters when the particles are tile-sorted
     int scatter = MPM_SCATTER_OWNED;  // tile scatter form (MPM_OPT_SCATTER)
     bool fused_remap = true;          // one-pass MUSL re-map (MPM_OPT_FUSED_REMAP)
+    bool tile_partials = true;        // tile scatters store windows (MPM_OPT_TILE_PARTIALS)
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> pool;
     // captured step graphs of mpm_run_graph, keyed by what their kernel
@@ -206,6 +207,7 @@ struct Impl : ImplBase {
     GridPtrs<R> gp{};
     Material<R>* d_mats = nullptr;
     double* d_stage = nullptr;  // np doubles, download staging
+    R* tile_part = nullptr;     // tile partials: ntiles x (1 + 2 dim) x window nodes
     std::vector<void*> owned;
     SortState<R> sort;
     // learned 3D Mohr-Coulomb return (FP32 only)
@@ -336,6 +338,12 @@ struct Impl : ImplBase {
                 }
             });
             if (ea != hipSuccess) return c->fail(MPM_E_NOMEM, std::string("sort buffers: ") + hipGetErrorString(ea));
+            // tile partials of the widest scatter (P2G #1: 1 + 2 dim fields)
+            const int S = c->cfg.basis == MPM_LINEAR ? 2 : 3;
+            const long long ww = (1ll << sort.tg.tsh) + S - 1;
+            const long long wn = dim == 3 ? ww * ww * ww : ww * ww;
+            const long long cnt = (long long)sort.tg.ntiles * (1 + 2 * dim) * wn;
+            if ((rc = alloc(c, &tile_part, cnt, "tile partials")) != MPM_OK) return rc;
         }
         return MPM_OK;
     }
@@ -533,6 +541,21 @@ struct Impl : ImplBase {
         const bool tiled = sort.enabled && sort.tiles_valid && c->use_tiles;
         const bool owned = c->scatter != MPM_SCATTER_ROUNDS;
         constexpr int TSH = DIM == 3 ? 2 : 3;
+        // tile partials: the tile scatters store their windows, the grid solves
+        // add them up (no flush atomics; MPM_OPT_TILE_PARTIALS)
+        R* const tbp = (tiled && c->tile_partials) ? tile_part : nullptr;
+        TilePartials<R> tpart{};
+        if (tbp) {
+            constexpr int WWt = Win<DIM, S, TSH>::WW;
+            tpart.tb = tbp;
+            tpart.slot_of = sort.slot_of;
+            tpart.tsh = TSH;
+            tpart.ww = WWt;
+            tpart.wn = Win<DIM, S, TSH>::WN;
+            tpart.tn[0] = sort.tg.tx;
+            tpart.tn[1] = sort.tg.ty;
+            tpart.tn[2] = sort.tg.tz;
+        }
         // Base-owned scatter passes (OwnedPasses in mpm_owned.hpp): FP64 3D
         // always takes one z layer per pass (its FP64 sums would not fit the
         // register file); FP32 3D P2G #1 sums mass and momentum, then force
@@ -546,11 +569,11 @@ struct Impl : ImplBase {
             if (DIM == 3 && (F64 || layers))
                 k_scatter_owned<DIM, S, TSH, R, MODE, DIM == 3 ? 2 : 0><<<sort.tiled_grid(), 64, 0, s>>>(
                     geom, sort.tg, pp, gp, Gv, sort.masks, sort.offs, sort.ovf, sort.rounds,
-                    sort.tile_list, sort.seg_begin, sort.seg_end, sort.tile_count, fl);
+                    sort.tile_list, sort.seg_begin, sort.seg_end, sort.tile_count, fl, tbp);
             else if constexpr (!F64 || DIM == 2)
                 k_scatter_owned<DIM, S, TSH, R, MODE, DIM == 3 ? SP_FIELDS : 0><<<sort.tiled_grid(), 64, 0, s>>>(
                     geom, sort.tg, pp, gp, Gv, sort.masks, sort.offs, sort.ovf, sort.rounds,
-                    sort.tile_list, sort.seg_begin, sort.seg_end, sort.tile_count, fl);
+                    sort.tile_list, sort.seg_begin, sort.seg_end, sort.tile_count, fl, tbp);
         };
         if (phase != PH_HEAD) {
             if ((rc = c->launch(KID_P2G, [&] {
@@ -559,13 +582,13 @@ struct Impl : ImplBase {
                      else if (tiled)
                          k_scatter_tiled<DIM, S, TSH, R, 0><<<sort.tiled_grid(), 64, 0, s>>>(
                              geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
-                             sort.seg_begin, sort.seg_end, sort.tile_count, fl);
+                             sort.seg_begin, sort.seg_end, sort.tile_count, fl, tbp);
                      else
                          k_p2g<DIM, S, R><<<pb, kBlock, 0, s>>>(geom, pp, gp, Gv, n, fl);
                  })) != MPM_OK)
                 return rc;
             if ((rc = c->launch(KID_GRID1, [&] {
-                     k_grid1<DIM, R><<<gb, kBlock, 0, s>>>(geom, gp, sa, int(nc));
+                     k_grid1<DIM, R><<<gb, kBlock, 0, s>>>(geom, gp, sa, int(nc), tpart);
                  })) != MPM_OK)
                 return rc;
             // re-map gather (tiled: shared window, 256 threads); MUSL then scatters the
@@ -581,12 +604,12 @@ struct Impl : ImplBase {
                                  k_remap_owned<DIM, S, TSH, R, true><<<sort.tiled_grid(), 64, 0, s>>>(
                                      geom, sort.tg, pp, gp, sa, sort.masks, sort.offs, sort.ovf,
                                      sort.rounds, sort.tile_list, sort.seg_begin, sort.seg_end,
-                                     sort.tile_count);
+                                     sort.tile_count, tbp);
                              else
                                  k_remap_owned<DIM, S, TSH, R, false><<<sort.tiled_grid(), 64, 0, s>>>(
                                      geom, sort.tg, pp, gp, sa, sort.masks, sort.offs, sort.ovf,
                                      sort.rounds, sort.tile_list, sort.seg_begin, sort.seg_end,
-                                     sort.tile_count);
+                                     sort.tile_count, tbp);
                          }
                      } else if (tiled)
                          if (sa.va != 0 || sa.pic != R(0))
@@ -608,13 +631,13 @@ struct Impl : ImplBase {
                          else
                              k_scatter_tiled<DIM, S, TSH, R, 1><<<sort.tiled_grid(), 64, 0, s>>>(
                                  geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
-                                 sort.seg_begin, sort.seg_end, sort.tile_count, fl);
+                                 sort.seg_begin, sort.seg_end, sort.tile_count, fl, tbp);
                      })) != MPM_OK)
                     return rc;
             }
             if (sa.remap) {
                 if ((rc = c->launch(KID_GRID2, [&] {
-                         k_grid2<DIM, R><<<nblocks(ni), kBlock, 0, s>>>(geom, gp, R(dt));
+                         k_grid2<DIM, R><<<nblocks(ni), kBlock, 0, s>>>(geom, gp, R(dt), tpart);
                      })) != MPM_OK)
                     return rc;
             }
@@ -808,6 +831,8 @@ struct Impl : ImplBase {
         if (m && !have_de) {
             for (int k = 0; k < 6; ++k)
                 if ((rc = alloc(c, &pp.de[k], np, "strain increment")) != MPM_OK) return rc;
+            for (int k = 0; k < 6; ++k)
+                if ((rc = alloc(c, &pp.pr[k], np, "trial principal stresses and frame")) != MPM_OK) return rc;
             have_de = true;
         }
         if (!d_sur_stats && (rc = alloc(c, &d_sur_stats, 2, "surrogate stats")) != MPM_OK) return rc;
@@ -1177,6 +1202,8 @@ int mpm_set_option(mpm_ctx* c, int option, int64_t value) {
         c->use_tiles = value != 0;
     } else if (option == MPM_OPT_FUSED_REMAP) {
         c->fused_remap = value != 0;
+    } else if (option == MPM_OPT_TILE_PARTIALS) {
+        c->tile_partials = value != 0;
     } else if (option == MPM_OPT_SCATTER) {
         if (value != MPM_SCATTER_ROUNDS && value != MPM_SCATTER_OWNED && value != MPM_SCATTER_OWNED_LAYERS)
             return c->fail(MPM_E_ARG, "mpm_set_option: scatter form is MPM_SCATTER_ROUNDS, "

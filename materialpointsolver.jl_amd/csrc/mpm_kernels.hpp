@@ -88,6 +88,7 @@ struct PartPtrs {
     int* p2c;         // step-start cell id (MVL only)
     R* de[6];         // strain increment of the step (deferred 3D Mohr-Coulomb return only)
     R* pm[3];         // MUSL scratch: new momentum between the re-map's gather and scatter
+    R* pr[6];         // learned return: trial principal stresses (3) and rotation (3)
 };
 
 template <typename R>
@@ -149,6 +150,63 @@ struct StepArgs {
     unsigned long long* log_count;
     long long log_cap;
 };
+
+// Tile partials of a scatter (the tile kernels' windows stored per tile, see
+// flush_window): tb[slot][f][k] with the tile's slot from slot_of (-1: no
+// particles), windows of ww nodes per axis starting at tile * 2^tsh.  tb ==
+// nullptr: the scatter added atomically into the grid fields.
+template <typename R>
+struct TilePartials {
+    const R* tb;
+    const int* slot_of;
+    int tsh, ww, wn;
+    int tn[3];  // tiles per axis
+};
+
+// Adds to acc[0, NF) the partials of node n from every tile window that holds
+// it, in a fixed order (tiles with lower indices first).
+template <int DIM, int NF, typename R>
+__device__ __forceinline__ void add_partials(const GridGeom& g, const TilePartials<R>& tp,
+                                             long long n, R* acc) {
+    // node coordinates with 32-bit divisions (grids stay below 2^31 nodes);
+    // 64-bit divisions here cost more than the rest of the grid solve
+    const unsigned nx = unsigned(g.nn[0]), ny = unsigned(g.nn[1]);
+    const unsigned un = unsigned(n);
+    const unsigned row = un / nx;
+    int i[3];
+    i[0] = int(un - row * nx);
+    if constexpr (DIM == 3) {
+        const unsigned lay = row / ny;
+        i[1] = int(row - lay * ny);
+        i[2] = int(lay);
+    } else {
+        i[1] = int(row);
+        i[2] = 0;
+    }
+    int lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (a < DIM) {
+            // windows [t 2^tsh, t 2^tsh + ww - 1] that contain i
+            hi[a] = min(i[a] >> tp.tsh, tp.tn[a] - 1);
+            const int l = i[a] - tp.ww + 1;
+            lo[a] = l <= 0 ? 0 : ((l + (1 << tp.tsh) - 1) >> tp.tsh);
+        } else {
+            lo[a] = hi[a] = 0;
+        }
+    }
+    for (int tz = lo[2]; tz <= hi[2]; ++tz)
+        for (int ty = lo[1]; ty <= hi[1]; ++ty)
+            for (int tx = lo[0]; tx <= hi[0]; ++tx) {
+                const int s = tp.slot_of[tx + tp.tn[0] * (ty + tp.tn[1] * tz)];
+                if (s < 0) continue;
+                const int k = (i[0] - (tx << tp.tsh)) +
+                              tp.ww * ((i[1] - (ty << tp.tsh)) + (DIM == 3 ? tp.ww * (i[2] - (tz << tp.tsh)) : 0));
+                const R* b = tp.tb + (long long)s * NF * tp.wn + k;
+#pragma unroll
+                for (int f = 0; f < NF; ++f) acc[f] += b[f * tp.wn];
+            }
+}
 
 // ---------------------------------------------------------------------------
 // Basis functions
@@ -419,27 +477,37 @@ __global__ __launch_bounds__(256) void k_p2g(GridGeom g, PartPtrs<R> pp, GridPtr
 // ---------------------------------------------------------------------------
 template <int DIM, typename R>
 __global__ __launch_bounds__(256) void k_grid1(GridGeom g, GridPtrs<R> gp, StepArgs<R> sa,
-                                               int ncells) {
+                                               int ncells, TilePartials<R> tp) {
     long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n < ncells && sa.mvl) {
         gp.cell_sm[n] = R(0);
         gp.cell_vol[n] = R(0);
     }
     if (n >= g.ni) return;
-    const R m = gp.m[n];
+    // the grid fields hold what was added atomically (every P2G without tile
+    // partials; with them, the particles that left their tile's window)
+    R m = gp.m[n];
+    R acc[1 + 2 * DIM];
+#pragma unroll
+    for (int f = 0; f < 1 + 2 * DIM; ++f) acc[f] = R(0);
+    if (tp.tb) {
+        add_partials<DIM, 1 + 2 * DIM, R>(g, tp, n, acc);
+        m += acc[0];
+    }
     // A node without mass received nothing from P2G (every particle with a
     // non-zero weight on it adds m_p N > 0), so p and f are still zero, and no
     // gather reads it (gathers skip zero weights): nothing to do.
     if (m == R(0)) return;
+    if (tp.tb && sa.remap) gp.m[n] = m;  // for grid solve #2
     const R msd = m < R(kEpsF64) ? R(0) : R(1) / m;
     const unsigned bc = gp.bcf[n];
     R P[3], Fc[3], vs[3];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) P[d] = gp.p[d][n];
+    for (int d = 0; d < DIM; ++d) P[d] = gp.p[d][n] + acc[1 + d];
     R ss = R(0);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        R fd = gp.f[d][n];
+        R fd = gp.f[d][n] + acc[1 + DIM + d];
         Fc[d] = fd;
         ss += fd * fd;
     }
@@ -586,18 +654,20 @@ __global__ __launch_bounds__(256) void k_g2p1_p2g2(GridGeom g, PartPtrs<R> pp, G
 // Grid solve #2 (doublemapping3_OS!, utils_OS.jl:968-1016); clears m and p.
 // ---------------------------------------------------------------------------
 template <int DIM, typename R>
-__global__ __launch_bounds__(256) void k_grid2(GridGeom g, GridPtrs<R> gp, R dt) {
+__global__ __launch_bounds__(256) void k_grid2(GridGeom g, GridPtrs<R> gp, R dt, TilePartials<R> tp) {
     long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.ni) return;
     const R m = gp.m[n];
     if (m == R(0)) return;  // no mass: p is zero and no G2P weight reads du here
     const R msd = m < R(kEpsF64) ? R(0) : R(1) / m;
     const unsigned bc = gp.bcf[n];
+    R acc[3] = {0, 0, 0};
+    if (tp.tb) add_partials<DIM, DIM, R>(g, tp, n, acc);
     // loads before stores (a load after a store to another field waits for it)
     R P[3], bv[3];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        P[d] = gp.p[d][n];
+        P[d] = gp.p[d][n] + acc[d];
         bv[d] = (bc & (1u << d)) ? gp.bcv[d][n] : R(0);
     }
 #pragma unroll
@@ -894,6 +964,152 @@ __device__ __forceinline__ MCStrength<R> mc_strength(const Material<R>& M, R eq)
     return p;
 }
 
+// Rotation (eigenvectors in the columns of V, any signs) <-> unit quaternion
+// (x, y, z; w >= 0 is implied).  A reflection is made a rotation by turning
+// the last column, which leaves V diag(p) V^T unchanged.
+template <typename R>
+__device__ __forceinline__ void quat_from_eigvecs(R (*V)[3], R* q3) {
+    const R det = V[0][0] * (V[1][1] * V[2][2] - V[1][2] * V[2][1]) -
+                  V[0][1] * (V[1][0] * V[2][2] - V[1][2] * V[2][0]) +
+                  V[0][2] * (V[1][0] * V[2][1] - V[1][1] * V[2][0]);
+    if (det < R(0))
+#pragma unroll
+        for (int r = 0; r < 3; ++r) V[r][2] = -V[r][2];
+    const R tr = V[0][0] + V[1][1] + V[2][2];
+    R w, x, y, z;
+    if (tr > R(0)) {
+        const R s = sqrt(tr + R(1)) * R(2);
+        w = R(0.25) * s;
+        x = (V[2][1] - V[1][2]) / s;
+        y = (V[0][2] - V[2][0]) / s;
+        z = (V[1][0] - V[0][1]) / s;
+    } else if (V[0][0] > V[1][1] && V[0][0] > V[2][2]) {
+        const R s = sqrt(R(1) + V[0][0] - V[1][1] - V[2][2]) * R(2);
+        w = (V[2][1] - V[1][2]) / s;
+        x = R(0.25) * s;
+        y = (V[0][1] + V[1][0]) / s;
+        z = (V[0][2] + V[2][0]) / s;
+    } else if (V[1][1] > V[2][2]) {
+        const R s = sqrt(R(1) + V[1][1] - V[0][0] - V[2][2]) * R(2);
+        w = (V[0][2] - V[2][0]) / s;
+        x = (V[0][1] + V[1][0]) / s;
+        y = R(0.25) * s;
+        z = (V[1][2] + V[2][1]) / s;
+    } else {
+        const R s = sqrt(R(1) + V[2][2] - V[0][0] - V[1][1]) * R(2);
+        w = (V[1][0] - V[0][1]) / s;
+        x = (V[0][2] + V[2][0]) / s;
+        y = (V[1][2] + V[2][1]) / s;
+        z = R(0.25) * s;
+    }
+    const R sg = w < R(0) ? R(-1) : R(1);
+    q3[0] = sg * x;
+    q3[1] = sg * y;
+    q3[2] = sg * z;
+}
+
+template <typename R>
+__device__ __forceinline__ void eigvecs_from_quat(const R* q3, R (*V)[3]) {
+    const R x = q3[0], y = q3[1], z = q3[2];
+    const R ww = R(1) - (x * x + y * y + z * z);
+    const R w = ww > R(0) ? sqrt(ww) : R(0);
+    V[0][0] = R(1) - R(2) * (y * y + z * z);
+    V[0][1] = R(2) * (x * y - z * w);
+    V[0][2] = R(2) * (x * z + y * w);
+    V[1][0] = R(2) * (x * y + z * w);
+    V[1][1] = R(1) - R(2) * (x * x + z * z);
+    V[1][2] = R(2) * (y * z - x * w);
+    V[2][0] = R(2) * (x * z - y * w);
+    V[2][1] = R(2) * (y * z + x * w);
+    V[2][2] = R(1) - R(2) * (x * x + y * y);
+}
+
+// Inputs of the learned 3D Mohr-Coulomb return (16): the trial stress as its
+// principal values p (descending) over the scale S = max|p| + c cos(phi) and
+// its principal frame as a quaternion (3), the strain increment (Voigt,
+// engineering shears) times 2G / S, the accumulated plastic strain, and the
+// strength in effect (phi, c / S, psi).  Scale-free where the return is.
+template <typename R>
+__device__ __forceinline__ void mc_features(const R* p, const R* q3, const R* de, R eq,
+                                            const MCStrength<R>& P, R Gs, float* x) {
+    const R ap0 = p[0] < R(0) ? -p[0] : p[0], ap2 = p[2] < R(0) ? -p[2] : p[2];
+    const R S = (ap0 > ap2 ? ap0 : ap2) + P.c * P.cosphi + R(1e-30);
+    const R iS = R(1) / S;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        x[k] = float(p[k] * iS);
+        x[3 + k] = float(q3[k]);
+    }
+    const R ge = R(2) * Gs * iS;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) x[6 + k] = float(de[k] * ge);
+    x[12] = float(eq);
+    x[13] = float(P.phi);
+    x[14] = float(P.c * iS);
+    x[15] = float(P.psi);
+}
+
+// The principal-space return on one given branch (1 main plane, 2 edge
+// s1 = s2, 3 edge s2 = s3, 4 apex) of a yielding trial stress p, accepted
+// only when that branch's own conditions hold: the main-plane return keeps
+// the principal order; an edge return has non-negative multipliers of both
+// planes and keeps s1 >= s3 (mc3_principal's edge test); the apex is taken
+// only where mc3_principal itself ends at the apex.  Returns true with p the
+// returned stresses, false with p unchanged.  The closed forms are
+// mc3_principal's, so a main-plane or apex branch that is accepted gives its
+// stresses exactly; an accepted edge is the one whose conditions hold.
+template <typename R>
+__device__ __forceinline__ bool mc3_principal_branch(R* p, int kind, R Dt, R Dd, R sphi, R cphi,
+                                                     R spsi, R c) {
+    const R k2 = R(2) * c * cphi;
+    auto dmul = [&](const R* n, R* out) {
+        out[0] = Dt * n[0] + Dd * n[1] + Dd * n[2];
+        out[1] = Dd * n[0] + Dt * n[1] + Dd * n[2];
+        out[2] = Dd * n[0] + Dd * n[1] + Dt * n[2];
+    };
+    auto dot = [](const R* a, const R* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    const R f = (p[0] - p[2]) + (p[0] + p[2]) * sphi - k2;
+    const R Ga[3] = {R(1) + sphi, R(0), -(R(1) - sphi)};
+    const R Na[3] = {R(1) + spsi, R(0), -(R(1) - spsi)};
+    R DNa[3], q[3];
+    dmul(Na, DNa);
+    const R a11 = dot(Ga, DNa);
+    if (kind == 1) {
+        const R g = f / a11;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) q[k] = p[k] - g * DNa[k];
+        if (!(q[0] >= q[1] && q[1] >= q[2])) return false;
+    } else if (kind == 2 || kind == 3) {
+        R Gb[3], Nb[3], DNb[3], fb;
+        if (kind == 2) {  // s1 = s2: second plane (s2 - s3)
+            Gb[0] = R(0); Gb[1] = R(1) + sphi; Gb[2] = -(R(1) - sphi);
+            Nb[0] = R(0); Nb[1] = R(1) + spsi; Nb[2] = -(R(1) - spsi);
+            fb = (p[1] - p[2]) + (p[1] + p[2]) * sphi - k2;
+        } else {          // s2 = s3: second plane (s1 - s2)
+            Gb[0] = R(1) + sphi; Gb[1] = -(R(1) - sphi); Gb[2] = R(0);
+            Nb[0] = R(1) + spsi; Nb[1] = -(R(1) - spsi); Nb[2] = R(0);
+            fb = (p[0] - p[1]) + (p[0] + p[1]) * sphi - k2;
+        }
+        dmul(Nb, DNb);
+        const R a12 = dot(Ga, DNb), a21 = dot(Gb, DNa), a22 = dot(Gb, DNb);
+        const R det = a11 * a22 - a12 * a21;
+        if (!(det > R(0))) return false;
+        const R ga = (a22 * f - a12 * fb) / det;
+        const R gb = (a11 * fb - a21 * f) / det;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) q[k] = p[k] - ga * DNa[k] - gb * DNb[k];
+        if (!((ga >= R(0) && gb >= R(0) && q[0] >= q[2]) || !(sphi > R(0)))) return false;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) q[k] = p[k];
+        if (mc3_principal<R>(q, Dt, Dd, sphi, cphi, spsi, c) != 4) return false;
+    }
+    p[0] = q[0];
+    p[1] = q[1];
+    p[2] = q[2];
+    return true;
+}
+
 // Plastic strain bookkeeping of a stress correction d = s_new - s_trial
 // (inverse elastic matrix; ϵk assigned, ϵq accumulated).
 template <typename R>
@@ -933,27 +1149,25 @@ __device__ __forceinline__ int mcP3(R* s, R& eq, R& ek, const Material<R>& M) {
     return kind;
 }
 
-// Appends one return-mapping sample (raw units): inputs trial stress, strain
-// increment, ϵq, ϕ, cohesion in effect, ψ; outputs stress correction, Δϵq.
+// Appends one return-mapping sample: the learned return's 16 inputs
+// (mc_features of the trial stress s0, strain increment e, plastic strain eq0
+// and the strength in effect) and its 7 targets: the branch the exact return
+// took as a one-hot over (main plane, edge s1 = s2, edge s2 = s3, apex) and
+// three zeros.
 template <typename R>
 __device__ __forceinline__ void log_return(const StepArgs<R>& sa, const R* s0, const R* e, R eq0,
-                                           const R* s1, R eq1, const Material<R>& M) {
+                                           int kind, const Material<R>& M) {
     const unsigned long long slot = atomicAdd(sa.log_count, 1ull);
     if ((long long)slot >= sa.log_cap) return;
     float* x = sa.log_x + slot * 16;
     float* y = sa.log_y + slot * 7;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        x[k] = float(s0[k]);
-        x[6 + k] = float(e[k]);
-        y[k] = float(s1[k] - s0[k]);
-    }
+    R lam[3], V[3][3], q3[3];
+    sym3_eig<R>(s0, lam, V);
+    quat_from_eigvecs<R>(V, q3);
     const MCStrength<R> P = mc_strength<R>(M, eq0);
-    x[12] = float(eq0);
-    x[13] = float(P.phi);
-    x[14] = float(P.c);
-    x[15] = float(P.psi);
-    y[6] = float(eq1 - eq0);
+    mc_features<R>(lam, q3, e, eq0, P, M.Gs, x);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) y[k] = (k == kind - 1) ? 1.0f : 0.0f;
 }
 
 // Cell id of the reference's p2c (fld on x and z, cld on y), 0-based, in this
@@ -1086,15 +1300,26 @@ __device__ __forceinline__ void g2p2_commit(const GridGeom& g, PartPtrs<R>& pp, 
             mcP2<R>(s, eq, ek, M);
         } else {
             if (sa.defer_return) {
-                // trial stress is stored below; the learned-return kernel finishes
+                // trial stress (stored below), strain increment and the trial
+                // stress's principal values and frame: the learned-return
+                // kernel finishes without an eigen-solve of its own
 #pragma unroll
                 for (int k = 0; k < 6; ++k) pp.de[k][i] = e[k];
+                R lam[3], V[3][3], q3[3];
+                sym3_eig<R>(s, lam, V);
+                quat_from_eigvecs<R>(V, q3);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    pp.pr[k][i] = lam[k];
+                    pp.pr[3 + k][i] = q3[k];
+                }
             } else if (sa.log_x && i % sa.log_stride == 0) {
                 R s0[6];
 #pragma unroll
                 for (int k = 0; k < 6; ++k) s0[k] = s[k];
                 const R eq0 = eq;
-                if (mcP3<R>(s, eq, ek, M) != 0) log_return<R>(sa, s0, e, eq0, s, eq, M);
+                const int kind = mcP3<R>(s, eq, ek, M);
+                if (kind != 0) log_return<R>(sa, s0, e, eq0, kind, M);
             } else {
                 mcP3<R>(s, eq, ek, M);
             }

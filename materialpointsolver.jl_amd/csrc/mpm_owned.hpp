@@ -492,7 +492,8 @@ __global__ __launch_bounds__(64) void k_scatter_owned(
     const unsigned long long* __restrict__ masks, const int* __restrict__ offs,
     const int* __restrict__ ovf, const int* __restrict__ rounds,
     const int* __restrict__ tile_list, const int* __restrict__ seg_begin,
-    const int* __restrict__ seg_end, const int* __restrict__ count_ptr, int* flags) {
+    const int* __restrict__ seg_end, const int* __restrict__ count_ptr, int* flags,
+    R* __restrict__ tb) {
     using W = Win<DIM, S, TSH>;
     using PS = OwnedPasses<DIM, S, MODE, SPLIT>;
     static_assert(kWinSlack == 0, "lane-owned bases assume a slack-free window");
@@ -588,7 +589,8 @@ __global__ __launch_bounds__(64) void k_scatter_owned(
             owned_add_sums<DIM, S, NF, F0, F1, Z0, NZP, R, WW, ZS, WN, PAIRED>(win, wbase0, acc, any);
         });
         drift_all();
-        flush_window<DIM, WW, ZS, WN, NF, R>(g, w0, win, fields);
+        flush_window<DIM, WW, ZS, WN, NF, R>(g, w0, win, fields,
+                                             tb ? tb + (long long)e * NF * WN : nullptr);
     }
     raise_flag(flags, flag);
 }
@@ -610,7 +612,7 @@ __global__ __launch_bounds__(64) void k_remap_owned(
     const unsigned long long* __restrict__ masks, const int* __restrict__ offs,
     const int* __restrict__ ovf, const int* __restrict__ rounds,
     const int* __restrict__ tile_list, const int* __restrict__ seg_begin,
-    const int* __restrict__ seg_end, const int* __restrict__ count_ptr) {
+    const int* __restrict__ seg_end, const int* __restrict__ count_ptr, R* __restrict__ tb) {
     using W = Win<DIM, S, TSH>;
     static_assert(kWinSlack == 0, "lane-owned bases assume a slack-free window");
     static_assert(W::NBN == 64, "one base per lane");
@@ -744,7 +746,8 @@ __global__ __launch_bounds__(64) void k_remap_owned(
         owned_add_sums<DIM, S, NF, 0, NF, 0, DIM == 3 ? S : 1, R, WW, ZS, WN, PAIRED>(win, wbase0,
                                                                                   acc, any);
         drift_all();
-        flush_window<DIM, WW, ZS, WN, NF, R>(g, w0, win, fields);
+        flush_window<DIM, WW, ZS, WN, NF, R>(g, w0, win, fields,
+                                             tb ? tb + (long long)e * NF * WN : nullptr);
     }
 }
 

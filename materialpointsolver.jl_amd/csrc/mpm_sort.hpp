@@ -139,6 +139,29 @@ __global__ __launch_bounds__(256) void k_tile_segments(const int* __restrict__ s
     nonempty[t] = totals[t] > 0 ? 1 : 0;
 }
 
+// slot_of[t]: the position of tile t in the compacted tile list (the tile
+// partials' slot), -1 for an empty tile.  The list holds the non-empty tiles
+// in increasing order, so tile t's position is the number of non-empty tiles
+// below it; one thread per tile finds it by binary search in the list.
+__global__ __launch_bounds__(256) void k_tile_slots(const unsigned char* __restrict__ nonempty,
+                                                    const int* __restrict__ tile_list,
+                                                    const int* __restrict__ count_ptr,
+                                                    int* __restrict__ slot_of, int ntiles) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    if (!nonempty[t]) {
+        slot_of[t] = -1;
+        return;
+    }
+    int lo = 0, hi = *count_ptr;  // tile_list[lo .. hi) is sorted; t is in it
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (tile_list[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    slot_of[t] = lo;
+}
+
 // Destination of every particle in the sorted order: order[pos] = i and the
 // sorted key at pos.
 __global__ __launch_bounds__(256) void k_sort_dest(const unsigned* __restrict__ keys,
@@ -216,6 +239,7 @@ struct SortState {
     unsigned long long* masks = nullptr;
     int *offs = nullptr, *ovf = nullptr, *totals = nullptr, *starts = nullptr;
     int* rounds = nullptr;  // per tile: the largest particle count of one base
+    int* slot_of = nullptr;  // per tile: its place in tile_list, -1 when empty
     void* temp = nullptr;
     size_t temp_bytes = 0;
     // Alternate buffers, one per permuted stream in for_each_stream order;
@@ -268,6 +292,7 @@ struct SortState {
         if (e == hipSuccess) e = hipMalloc((void**)&totals, sizeof(int) * nt);
         if (e == hipSuccess) e = hipMalloc((void**)&starts, sizeof(int) * nt);
         if (e == hipSuccess) e = hipMalloc((void**)&rounds, sizeof(int) * nt);
+        if (e == hipSuccess) e = hipMalloc((void**)&slot_of, sizeof(int) * nt);
         if (e != hipSuccess) {
             error = std::string("hipMalloc: ") + hipGetErrorString(e);
             return MPM_E_NOMEM;
@@ -289,13 +314,13 @@ struct SortState {
 
     void release() {
         void* ps[] = {keys, keys_alt, idx, idx_alt, temp, seg_begin, seg_end, tile_list,
-                      tile_count, nonempty, count, masks, offs, ovf, totals, starts, rounds};
+                      tile_count, nonempty, count, masks, offs, ovf, totals, starts, rounds, slot_of};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         keys = keys_alt = count = nullptr;
         idx = idx_alt = nullptr;
         masks = nullptr;
-        offs = ovf = totals = starts = rounds = nullptr;
+        offs = ovf = totals = starts = rounds = slot_of = nullptr;
         temp = nullptr;
         seg_begin = seg_end = tile_list = tile_count = nullptr;
         nonempty = nullptr;
@@ -377,6 +402,11 @@ struct SortState {
         });
         if (rc != MPM_OK) return rc;
         if (lib != hipSuccess) return c->hip_fail(lib, "tile list compaction");
+        rc = c->launch(6, [&] {
+            const unsigned tb = unsigned((tg.ntiles + 255) / 256);
+            k_tile_slots<<<tb, 256, 0, s>>>(nonempty, tile_list, tile_count, slot_of, tg.ntiles);
+        });
+        if (rc != MPM_OK) return rc;
         tiles_valid = true;
         return MPM_OK;
     }

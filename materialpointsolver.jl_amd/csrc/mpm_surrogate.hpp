@@ -14,37 +14,39 @@
 
 namespace mpm {
 
-// Network inputs of particle p after a deferred 3D Mohr-Coulomb step: trial
-// stress, strain increment, ϵq, ϕ, cohesion in effect, ψ.
+// Network inputs of particle p after a deferred 3D Mohr-Coulomb step: the
+// mc_features of its trial stress (principal values and frame, stored by
+// G2P #2), strain increment, plastic strain and strength in effect.  The
+// kernel reads them in two halves of 8.
 struct SurrogateIn {
     PartPtrs<float> pp;
     const Material<float>* mats;
     __device__ void load(long long p, int half, float* f) const {
-        if (half == 0) {
+        float pr[3], q3[3], de[6], x[16];
 #pragma unroll
-            for (int k = 0; k < 6; ++k) f[k] = pp.sig[k][p];
-            f[6] = pp.de[0][p];
-            f[7] = pp.de[1][p];
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) f[k] = pp.de[2 + k][p];
-            const Material<float>& M = mats[pp.nid[p]];
-            const float eq = pp.eq[p];
-            const MCStrength<float> P = mc_strength<float>(M, eq);
-            f[4] = eq;
-            f[5] = P.phi;
-            f[6] = P.c;
-            f[7] = P.psi;
+        for (int k = 0; k < 3; ++k) {
+            pr[k] = pp.pr[k][p];
+            q3[k] = pp.pr[3 + k][p];
         }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) de[k] = pp.de[k][p];
+        const Material<float>& M = mats[pp.nid[p]];
+        const float eq = pp.eq[p];
+        mc_features<float>(pr, q3, de, eq, mc_strength<float>(M, eq), M.Gs, x);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) f[k] = x[8 * half + k];
     }
 };
 
-// The learned return of particle p with the yield-function residual check:
-// a yielding trial stress takes the network's correction (and its ϵq
-// increment) when the corrected stress lies on the Mohr-Coulomb surface to
-// |f| <= tol * (2 c cos ϕ + |σ1 + σ3| sin ϕ) and the ϵq increment is
-// non-negative; otherwise the exact return (mcP3).  Elastic trial stresses
-// stay as they are (ϵk = 0, as mcP3).
+// The learned return of particle p.  The network scores the four branches
+// of the principal-space return (main plane, edge s1 = s2, edge s2 = s3,
+// apex); a yielding trial stress takes the highest-scoring branch when that
+// branch's own conditions hold (mc3_principal_branch: the returned stress is
+// that branch's closed form, on the Mohr-Coulomb surface) and its yield
+// residual |f| <= tol (2 c cos(phi) + |s1 + s3| sin(phi)); otherwise the
+// exact return (mc3_principal) decides.  The stress is rebuilt in the frame
+// G2P #2 stored (no eigen-solve here), and the plastic strains follow from
+// the stress correction as in mcP3.  Elastic trial stresses stay (ϵk = 0).
 // stats[0] counts yielding particles, stats[1] fallbacks to the exact return.
 struct SurrogateOut {
     PartPtrs<float> pp;
@@ -52,41 +54,57 @@ struct SurrogateOut {
     float tol;
     unsigned long long* stats;
     __device__ void store(long long p, const float* y) const {
-        float s[6];
+        float pr[3], q3[3];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) s[k] = pp.sig[k][p];
+        for (int k = 0; k < 3; ++k) {
+            pr[k] = pp.pr[k][p];
+            q3[k] = pp.pr[3 + k][p];
+        }
         const Material<float> M = mats[pp.nid[p]];
         float eq = pp.eq[p], ek = 0.0f;
         const MCStrength<float> P = mc_strength<float>(M, eq);
-        const float c = P.c;
-        float lam[3], V[3][3];
-        sym3_eig<float>(s, lam, V);
-        const float f = (lam[0] - lam[2]) + (lam[0] + lam[2]) * P.sinphi - 2.0f * c * P.cosphi;
+        const float k2 = 2.0f * P.c * P.cosphi;
+        const float f = (pr[0] - pr[2]) + (pr[0] + pr[2]) * P.sinphi - k2;
         const bool plastic = f > 0.0f;
         bool fallback = false;
         if (plastic) {
-            float sp[6];
+            int kind = 1;
+            float best = y[0];
 #pragma unroll
-            for (int k = 0; k < 6; ++k) sp[k] = s[k] + y[k];
-            float lp[3], Vp[3][3];
-            sym3_eig<float>(sp, lp, Vp);
-            const float F = (lp[0] - lp[2]) + (lp[0] + lp[2]) * P.sinphi - 2.0f * c * P.cosphi;
-            float scale = 2.0f * c * P.cosphi + fabsf(lp[0] + lp[2]) * P.sinphi;
-            scale = fmaxf(scale, 1e-6f * (fabsf(lp[0]) + fabsf(lp[2])) + 1e-30f);
-            const bool ok = fabsf(F) <= tol * scale && y[6] >= 0.0f && isfinite(F) &&
-                            isfinite(y[6]);
+            for (int k = 1; k < 4; ++k)
+                if (y[k] > best) {
+                    best = y[k];
+                    kind = k + 1;
+                }
+            float pn[3] = {pr[0], pr[1], pr[2]};
+            bool ok = isfinite(best) &&
+                      mc3_principal_branch<float>(pn, kind, M.Dt, M.Dd, P.sinphi, P.cosphi, P.sinpsi, P.c);
             if (ok) {
-                float eq_unused = 0.0f;
-                mc_strain<float>(y, eq_unused, ek, M);  // ϵk from the stress correction
-                eq += y[6];                             // ϵq increment from the network
-#pragma unroll
-                for (int k = 0; k < 6; ++k) pp.sig[k][p] = sp[k];
-            } else {
-                fallback = true;
-                mcP3<float>(s, eq, ek, M);
-#pragma unroll
-                for (int k = 0; k < 6; ++k) pp.sig[k][p] = s[k];
+                const float F = (pn[0] - pn[2]) + (pn[0] + pn[2]) * P.sinphi - k2;
+                float scale = k2 + fabsf(pn[0] + pn[2]) * P.sinphi;
+                scale = fmaxf(scale, 1e-6f * (fabsf(pn[0]) + fabsf(pn[2])) + 1e-30f);
+                ok = fabsf(F) <= tol * scale;
             }
+            if (!ok) {
+                fallback = true;
+                pn[0] = pr[0];
+                pn[1] = pr[1];
+                pn[2] = pr[2];
+                mc3_principal<float>(pn, M.Dt, M.Dd, P.sinphi, P.cosphi, P.sinpsi, P.c);
+            }
+            float V[3][3];
+            eigvecs_from_quat<float>(q3, V);
+            const int ri[6] = {0, 1, 2, 0, 1, 0}, ci[6] = {0, 1, 2, 1, 2, 2};
+            float d[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const float sn = pn[0] * V[ri[k]][0] * V[ci[k]][0] + pn[1] * V[ri[k]][1] * V[ci[k]][1] +
+                                 pn[2] * V[ri[k]][2] * V[ci[k]][2];
+                const float s0 = pp.sig[k][p];
+                d[k] = sn - s0;
+                pp.sig[k][p] = sn;
+            }
+            mc_strain<float>(d, eq, ek, M);
             pp.eq[p] = eq;
         }
         pp.ek[p] = ek;

@@ -157,11 +157,23 @@ __device__ __forceinline__ int paired_offset(int f, int k, int wn) {
     return f < 2 * NP ? (f >> 1) * 2 * wn + 2 * k + (f & 1) : 2 * NP * wn + k;
 }
 
-// Flush a scatter window: global atomic adds of every non-zero node (only
-// window nodes inside the grid can be non-zero).
+// Flush a scatter window.  With a tile-partials slot tb (the tile's place in
+// the tile list), the window is stored there, field-major (tb[f WN + k]), and
+// the grid solve adds the partials of the up to 2^D windows that hold a node:
+// plain stores and a fixed summation order.  Without (tb == nullptr): global
+// atomic adds of every non-zero node (only window nodes inside the grid can be
+// non-zero).
 template <int DIM, int WW, int ZS, int WN, int NF, typename R>
 __device__ __forceinline__ void flush_window(const GridGeom& g, const int* w0, lds_ptr<R> win,
-                                             R* const* fields) {
+                                             R* const* fields, R* tb = nullptr) {
+    if (tb) {
+        for (int k = threadIdx.x; k < WN; k += blockDim.x) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                tb[f * WN + k] = win[kPairedP2G<DIM, NF, R, WN> ? paired_offset<NF>(f, k, WN) : f * WN + k];
+        }
+        return;
+    }
     for (int k = threadIdx.x; k < WN; k += blockDim.x) {
         int kx, ky, kz;
         if (!window_node<DIM, WW, ZS>(k, kx, ky, kz)) continue;
@@ -308,7 +320,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, Gravity<R> G,
     const unsigned* __restrict__ keys, const int* __restrict__ tile_list,
     const int* __restrict__ seg_begin, const int* __restrict__ seg_end,
-    const int* __restrict__ count_ptr, int* flags) {
+    const int* __restrict__ count_ptr, int* flags, R* __restrict__ tb) {
     using W = Win<DIM, S, TSH>;
     constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
     constexpr int NF = MODE == 0 ? 1 + 2 * DIM : DIM;
@@ -523,7 +535,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         if constexpr (NWIN == 2) {
             for (int k = threadIdx.x; k < NF * WN; k += 64) win[k] = win[k] + win[NF * WN + k];
         }
-        flush_window<DIM, WW, ZS, WN, NF, R>(g, w0, win, fields);
+        flush_window<DIM, WW, ZS, WN, NF, R>(g, w0, win, fields,
+                                             tb ? tb + (long long)e * NF * WN : nullptr);
     }
     raise_flag(flags, flag);
 }

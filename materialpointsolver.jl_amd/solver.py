@@ -212,7 +212,9 @@ class HIPModel:
     def set_option(self, option, value):
         """Runtime option: "sort_interval" (>= 1), "tiles" (bool), "scatter"
         (the tile scatter form: "owned", "owned_layers" or "rounds"),
-        "fused_remap" (bool: the one-pass MUSL re-map with the owned forms) or
+        "fused_remap" (bool: the one-pass MUSL re-map with the owned forms),
+        "tile_partials" (bool: tile windows stored and summed by the grid solves
+        instead of flushed with atomics) or
         "profile_kernels" (kernel names that profile_enable times; empty or
         None = every kernel)."""
         if option == "profile_kernels":
@@ -231,7 +233,8 @@ class HIPModel:
             self._check(self.lib.mpm_set_option(self.ctx, capi.MPM_OPT_SCATTER, capi.SCATTER[value]))
             return
         code = {"sort_interval": capi.MPM_OPT_SORT_INTERVAL, "tiles": capi.MPM_OPT_TILES,
-                "fused_remap": capi.MPM_OPT_FUSED_REMAP}[option]
+                "fused_remap": capi.MPM_OPT_FUSED_REMAP,
+                "tile_partials": capi.MPM_OPT_TILE_PARTIALS}[option]
         self._check(self.lib.mpm_set_option(self.ctx, code, int(value)))
 
     # -- learned 3D Mohr-Coulomb return ----------------------------------------

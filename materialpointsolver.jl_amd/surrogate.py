@@ -7,9 +7,13 @@ normalisation from a sample set, training on the GPU (the library's own
 forward and backward matrix-core kernels with Adam), inference, gradients,
 saving and loading (``.npz``).
 
-Sample layout (row-major FP32): inputs (n, 16) = trial stress [xx,yy,zz,xy,yz,xz],
-strain increment (same order), eps_q, friction angle, cohesion in effect,
-dilation angle; outputs (n, 7) = stress correction (6) and eps_q increment.
+Sample layout (row-major FP32): inputs (n, 16) = the trial stress as its
+principal values p1 >= p2 >= p3 over the scale S = max|p| + c cos(phi) and its
+principal frame as a unit quaternion (x, y, z; w >= 0), the strain increment
+[xx,yy,zz,xy,yz,xz] times 2 G / S, eps_q, friction angle, c / S, dilation
+angle (``mc_features``); outputs (n, 7) = scores of the four branches of the
+principal-space Mohr-Coulomb return (main plane, edge s1 = s2, edge s2 = s3,
+apex; training targets one-hot) and three zeros.
 """
 from __future__ import annotations
 
@@ -22,6 +26,62 @@ SHAPES = [("W1", (NH, NIN)), ("b1", (NH,)), ("W2", (NH, NH)), ("b2", (NH,)),
           ("W3", (NH, NH)), ("b3", (NH,)), ("W4", (NOUT, NH)), ("b4", (NOUT,))]
 NPARAM = sum(int(np.prod(s)) for _, s in SHAPES)
 NTOTAL = NPARAM + NIN + NIN + NOUT
+
+
+BRANCHES = ("main plane", "edge s1 = s2", "edge s2 = s3", "apex")
+
+
+def _quat_xyz(V):
+    """Unit quaternion (x, y, z; w >= 0) of rotation matrices V (n, 3, 3)."""
+    n = V.shape[0]
+    out = np.zeros((n, 4))
+    tr = V[:, 0, 0] + V[:, 1, 1] + V[:, 2, 2]
+    for k in range(n):
+        m = V[k]
+        if tr[k] > 0:
+            s = np.sqrt(tr[k] + 1.0) * 2
+            q = (0.25 * s, (m[2, 1] - m[1, 2]) / s, (m[0, 2] - m[2, 0]) / s, (m[1, 0] - m[0, 1]) / s)
+        elif m[0, 0] > m[1, 1] and m[0, 0] > m[2, 2]:
+            s = np.sqrt(1.0 + m[0, 0] - m[1, 1] - m[2, 2]) * 2
+            q = ((m[2, 1] - m[1, 2]) / s, 0.25 * s, (m[0, 1] + m[1, 0]) / s, (m[0, 2] + m[2, 0]) / s)
+        elif m[1, 1] > m[2, 2]:
+            s = np.sqrt(1.0 + m[1, 1] - m[0, 0] - m[2, 2]) * 2
+            q = ((m[0, 2] - m[2, 0]) / s, (m[0, 1] + m[1, 0]) / s, 0.25 * s, (m[1, 2] + m[2, 1]) / s)
+        else:
+            s = np.sqrt(1.0 + m[2, 2] - m[0, 0] - m[1, 1]) * 2
+            q = ((m[1, 0] - m[0, 1]) / s, (m[0, 2] + m[2, 0]) / s, (m[1, 2] + m[2, 1]) / s, 0.25 * s)
+        out[k] = q
+    out[out[:, 0] < 0] *= -1.0
+    return out[:, 1:]
+
+
+def mc_features(sig, de, eq, phi, c, psi, Gs):
+    """The learned return's 16 inputs (as the device computes them) for trial
+    stresses sig (n, 6, [xx,yy,zz,xy,yz,xz]), strain increments de (n, 6),
+    plastic strains eq (n,) and the strength in effect (phi, c, psi) and shear
+    modulus Gs (scalars or (n,)).  Returns (X (n, 16), p (n, 3) principal
+    stresses, descending)."""
+    sig = np.atleast_2d(np.asarray(sig, dtype=np.float64))
+    n = sig.shape[0]
+    T = np.zeros((n, 3, 3))
+    T[:, 0, 0], T[:, 1, 1], T[:, 2, 2] = sig[:, 0], sig[:, 1], sig[:, 2]
+    T[:, 0, 1] = T[:, 1, 0] = sig[:, 3]
+    T[:, 1, 2] = T[:, 2, 1] = sig[:, 4]
+    T[:, 0, 2] = T[:, 2, 0] = sig[:, 5]
+    w, V = np.linalg.eigh(T)
+    p = w[:, ::-1]
+    V = V[:, :, ::-1].copy()
+    V[np.linalg.det(V) < 0, :, 2] *= -1.0
+    q3 = _quat_xyz(V)
+    phi = np.broadcast_to(np.asarray(phi, dtype=np.float64), (n,))
+    c = np.broadcast_to(np.asarray(c, dtype=np.float64), (n,))
+    psi = np.broadcast_to(np.asarray(psi, dtype=np.float64), (n,))
+    Gs = np.broadcast_to(np.asarray(Gs, dtype=np.float64), (n,))
+    S = np.maximum(np.abs(p[:, 0]), np.abs(p[:, 2])) + c * np.cos(phi) + 1e-30
+    X = np.concatenate([p / S[:, None], q3, np.asarray(de, dtype=np.float64) * (2 * Gs / S)[:, None],
+                        np.asarray(eq, dtype=np.float64)[:, None] * np.ones((n, 1)),
+                        phi[:, None], (c / S)[:, None], psi[:, None]], axis=1)
+    return X.astype(np.float32), p
 
 
 def init_params(seed=0):

@@ -9,7 +9,7 @@
   ensemble against the oracle run of the same instance alone.
 * GRF slope (20,000,000 particles, h = 0.5 m): 2,000 damped K0 steps, then
   runout steps; mass, finiteness, K0 normal and shear stress against the
-  infinite-slope values rho g d cos(theta) and rho g d sin(theta) at depth d.
+  infinite-slope values -rho g d cos(theta) and rho g d sin(theta) at depth d.
 """
 import math
 
@@ -165,7 +165,10 @@ def test_slope_20m_k0_and_runout(mpm):
     sel = ((x0[:, 0] > fx0 + 40) & (x0[:, 0] < fx0 + fl - 40) & (x0[:, 1] > fy0 + 25) &
            (x0[:, 1] < fy0 + fw - 25) & (d > 2.0) & (d < 8.0))
     szz = k0["sigma"][sel, 2] / (-rho * g * math.cos(theta) * d[sel])
-    sxz = k0["sigma"][sel, 5] / (-rho * g * math.sin(theta) * d[sel])
+    # infinite slope in the slope frame (gravity g sin(theta) downslope along +x,
+    # -g cos(theta) along z): d sigma_zz / dz = rho g cos, d sigma_xz / dz =
+    # -rho g sin, both zero at the free surface
+    sxz = k0["sigma"][sel, 5] / (rho * g * math.sin(theta) * d[sel])
     print(f"K0 ({sel.sum()} interior particles): sigma_zz / (rho g d cos) median {np.median(szz):.3f}, "
           f"sigma_xz / (rho g d sin) median {np.median(sxz):.3f}", flush=True)
     assert 0.85 < np.median(szz) < 1.15

@@ -18,17 +18,20 @@ def _rel(a, b):
 
 def _compare(mpm, orc, scene, nsteps, tol, sort_interval=0, keys=("xi", "vs", "sigma", "eps_s",
                                                                    "eps_q", "omega"),
-             scatter=None, fused_remap=None):
+             scatter=None, fused_remap=None, tile_partials=None):
     args, grid, mp, attr, bc = scene
     print(f"case: dim={grid.dim} eps={mp.ϵ} np={mp.np} const={args.constitutive} va={args.va} "
           f"MVL={args.MVL} NIC={grid.NIC} steps={nsteps} sort={sort_interval} "
-          f"scatter={scatter or 'default'} fused_remap={fused_remap}", flush=True)
+          f"scatter={scatter or 'default'} fused_remap={fused_remap} "
+          f"tile_partials={tile_partials}", flush=True)
     model = mpm.HIPModel(grid, mp, attr, bc, sort_interval=sort_interval)
     try:
         if scatter is not None:
             model.set_option("scatter", scatter)
         if fused_remap is not None:
             model.set_option("fused_remap", fused_remap)
+        if tile_partials is not None:
+            model.set_option("tile_partials", tile_partials)
         sargs = model.step_args(args)
         model.run(sargs, float(args.ΔT), 0.0, nsteps)
         model.sync()
@@ -104,6 +107,20 @@ def test_fp64_scatter_forms(mpm, orc, scatter, sort):
              sort_interval=sort, scatter=scatter)
     _compare(mpm, orc, _c3(mpm, constitutive="druckerprager"), 100, 1e-9, sort_interval=sort,
              scatter=scatter)
+
+
+# Tile partials (windows stored per tile and summed by the grid solves) and the
+# atomic window flush, with every scatter form, against the oracle
+@pytest.mark.parametrize("partials", [True, False])
+@pytest.mark.parametrize("scatter", ["owned", "owned_layers", "rounds"])
+def test_tile_partials_and_atomic_flush(mpm, orc, scatter, partials):
+    for sort in (5, 400):
+        _compare(mpm, orc, _c2(mpm, constitutive="druckerprager", cohesion=200.0), 200, 1e-9,
+                 sort_interval=sort, scatter=scatter, tile_partials=partials)
+        _compare(mpm, orc, _c3(mpm, constitutive="druckerprager"), 100, 1e-9,
+                 sort_interval=sort, scatter=scatter, tile_partials=partials)
+    _compare(mpm, orc, _c3(mpm, basis="linear", scheme="USL"), 60, 1e-9, sort_interval=5,
+             scatter=scatter, tile_partials=partials)
 
 
 @pytest.mark.parametrize("fused", [True, False])

@@ -1,5 +1,7 @@
 """The learned return map's matrix-core kernels against the FP64 MLP oracle
-(oracle/mlp_oracle.py) and the build-defined 3D Mohr-Coulomb return.
+(oracle/mlp_oracle.py) and the build-defined 3D Mohr-Coulomb return.  The
+network scores the branches of the principal-space return; the step takes the
+best-scoring branch where its conditions hold (else the exact return).
 
 Tolerances: BF16 operands carry 8 significant bits (relative rounding 2^-9),
 so the forward pass is compared with an oracle that rounds the same operands
@@ -89,42 +91,57 @@ def test_gradient_matches_oracle(mpm, mlo):
         S.close()
 
 
-def _mc_samples(orc, n, seed):
-    """Return-mapping samples of the oracle's 3D Mohr-Coulomb over random trial
-    stresses and materials (plastic states only)."""
+def _mc_samples(mpm, orc, n, seed):
+    """Learned-return samples over random trial stresses and materials (plastic
+    states only): the 16 inputs (mc_features) and the oracle's return branch as
+    a one-hot over the first four outputs."""
     rng = np.random.default_rng(seed)
     K, G = 7.0e5, 3.2308e5
-    X, Y = [], []
-    while len(X) < n:
+    S, D, E, kinds, mats = [], [], [], [], []
+    while len(S) < n:
         s = rng.normal(-2e4, 1.2e4, 6)
         s[3:] = rng.normal(0, 6e3, 3)
-        de = rng.normal(0, 2e-4, 6)
         phi = math.radians(rng.uniform(20, 40))
+        psi = math.radians(rng.uniform(0, 8))
         c = float(np.exp(rng.normal(math.log(1e3), 0.5)))
-        psi = 0.0
         eq = float(abs(rng.normal(0, 0.02)))
-        out, eq2, ek, kind = orc.mc3_point(s, eq, K, G, phi, psi, c)
+        kind = orc.mc3_point(s, eq, K, G, phi, psi, c)[3]
         if kind == 0:
             continue
-        X.append(np.concatenate([s, de, [eq, phi, c, psi]]))
-        Y.append(np.concatenate([out - s, [eq2 - eq]]))
-    return np.array(X, np.float32), np.array(Y, np.float32)
+        S.append(s)
+        D.append(rng.normal(0, 2e-4, 6))
+        E.append(eq)
+        mats.append((phi, c, psi))
+        kinds.append(kind)
+    mats = np.array(mats)
+    X, _ = mpm.surrogate.mc_features(np.array(S), np.array(D), np.array(E), mats[:, 0], mats[:, 1],
+                                     mats[:, 2], G)
+    Y = np.zeros((n, 7), np.float32)
+    Y[np.arange(n), np.array(kinds) - 1] = 1.0
+    return X, Y
 
 
-def test_training_learns_the_return_map(mpm, orc):
-    X, Y = _mc_samples(orc, 24000, 1)
-    Xt, Yt = _mc_samples(orc, 3000, 2)
+def _accuracy(pred, Y):
+    return float(np.mean(np.argmax(pred[:, :4], 1) == np.argmax(Y[:, :4], 1)))
+
+
+def test_training_learns_the_return_branch(mpm, orc):
+    X, Y = _mc_samples(mpm, orc, 24000, 1)
+    Xt, Yt = _mc_samples(mpm, orc, 3000, 2)
+    freq = Y[:, :4].mean(0)
+    print("branch frequencies (main, edge 12, edge 23, apex):", np.round(freq, 3), flush=True)
     S = mpm.Surrogate(batch_capacity=2048, seed=0)
     try:
         S.fit_normalisation(X, Y)
-        err0 = _rel(S.forward(Xt), Yt)
+        acc0 = _accuracy(S.forward(Xt), Yt)
         hist = S.train(X, Y, steps=1500, batch=1024, lr=2e-3, seed=3)
-        err1 = _rel(S.forward(Xt), Yt)
-        print(f"held-out rel L2 of the correction: untrained {err0:.3f} -> trained {err1:.3f}; "
+        acc1 = _accuracy(S.forward(Xt), Yt)
+        print(f"held-out branch accuracy: untrained {acc0:.3f} -> trained {acc1:.3f}; "
               f"loss {hist[:20].mean():.4f} -> {hist[-20:].mean():.4f}", flush=True)
         assert np.all(np.isfinite(hist))
-        assert hist[-50:].mean() < 0.2 * hist[:50].mean()
-        assert err1 < 0.35 and err1 < 0.4 * err0
+        assert hist[-50:].mean() < 0.5 * hist[:50].mean()
+        # better than always answering the most frequent branch
+        assert acc1 > 0.9 and acc1 > freq.max() + 0.02
     finally:
         S.close()
 
@@ -136,6 +153,9 @@ def _mc_column(mpm, h, eps="FP32", nsteps=400):
 
 
 def test_return_log_matches_exact_return(mpm, orc):
+    # a logged sample's inputs hold the trial stress's principal values and c
+    # in units of S; the principal-space return is homogeneous in (p, c), so
+    # the oracle's return on them takes the logged branch
     args, grid, mp, attr, bc = _mc_column(mpm, 0.02)
     m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=10)
     try:
@@ -147,17 +167,20 @@ def test_return_log_matches_exact_return(mpm, orc):
     print(f"logged {len(X)} samples", flush=True)
     assert len(X) > 1000
     K, G = float(attr.Ks[0]), float(attr.Gs[0])
+    Dt, Dd = K + 1.333333 * G, K - 0.666667 * G
+    assert np.all(Y[:, 4:] == 0) and np.all(Y[:, :4].sum(1) == 1)
     rng = np.random.default_rng(0)
-    worst = 0.0
-    for r in rng.choice(len(X), 300, replace=False):
+    agree = 0
+    rows = rng.choice(len(X), 300, replace=False)
+    for r in rows:
         x = X[r].astype(np.float64)
-        out, eq1, ek, kind = orc.mc3_point(x[:6], x[12], K, G, x[13], x[15], x[14])
-        assert kind != 0
-        scale = np.abs(x[:6]).max() + x[14]
-        worst = max(worst, np.abs(Y[r, :6] - (out - x[:6])).max() / scale)
-        assert abs(Y[r, 6] - (eq1 - x[12])) <= 2e-3 * abs(eq1 - x[12]) + 1e-9
-    print(f"worst stress-correction difference / |σ|: {worst:.2e}", flush=True)
-    assert worst <= 1e-4
+        assert np.linalg.norm(x[3:6]) <= 1.0 + 1e-6  # a unit quaternion's vector part
+        phi, c, psi = x[13], x[14], x[15]
+        p, kind = orc.mc3_principal(x[:3], Dt, Dd, math.sin(phi), math.cos(phi), math.sin(psi), c)
+        agree += kind == int(np.argmax(Y[r, :4])) + 1
+    print(f"logged branch = oracle branch in {agree} of {len(rows)}", flush=True)
+    # FP32 features carry ~1e-7 of rounding: a state on a branch boundary may flip
+    assert agree >= len(rows) - 3
 
 
 def test_learned_return_in_the_step(mpm):
@@ -206,5 +229,6 @@ def test_learned_return_in_the_step(mpm):
     assert tr[0] > 0 and un[0] > 0
     # what the trained network gains: it serves yielding particles the untrained one cannot
     assert tr[1] / tr[0] < un[1] / un[0]
+    assert tr[1] / tr[0] < 0.5
     # and the run stays on the exact solution at the runout tolerance of FP32 mode
     assert abs(tr[2]) <= 5e-3
