@@ -227,6 +227,15 @@ int mpm_profile_kernel_count(void);
  * No reference counterpart. */
 int mpm_count_active_nodes(mpm_ctx *ctx, int64_t *count);
 
+/* mpm_count_active_nodes without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  No reference counterpart.  The node
+ * bitmap (one bit per node) and the counter live in one file-scope device
+ * array of 4 MiB + 8 B that each GPU keeps until the process ends; grids of
+ * more than 33,554,432 nodes are rejected with MPM_E_RANGE.  Every call on a
+ * GPU shares that array (it is cleared at the start of each call): calls that
+ * overlap on one GPU, from any context or thread, race. */
+int mpm_count_active_nodes_noalloc(mpm_ctx *ctx, int64_t *count);
 
 /* Triad bandwidth probe A = B + C on nbytes-per-array FP32 buffers, in GB/s
  * (10^9 B/s), best of `reps`.  Replaces Tpeak(Val(:ROCm))

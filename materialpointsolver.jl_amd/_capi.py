@@ -97,6 +97,7 @@ _PROTOS = {
     "mpm_surrogate_stats": (C.c_int, [_CTX, _I64, _I64]),
     "mpm_return_log": (C.c_int, [_CTX, C.c_int64, C.c_int64]),
     "mpm_return_log_read": (C.c_int, [_CTX, _F, _F, C.c_int64, _I64]),
+    "mpm_count_active_nodes_noalloc": (C.c_int, [_CTX, _I64]),
 }
 
 

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -53,10 +54,39 @@ struct ProfRec {
 
 }  // namespace
 
+// Scratch of mpm_count_active_nodes_noalloc: a bitmap of the grid nodes (one
+// bit per node, at most kNoAllocMaxNodes nodes) followed by a 64-bit counter.
+// Each GPU holds its own copy for the life of the process; host code reaches
+// it only through the per-device address from hipGetSymbolAddress
+// (g_scratch_dev), never through the array's name.
+constexpr long long kNoAllocMaxNodes = 1ll << 25;            // 33,554,432 nodes
+constexpr long long kNoAllocWords = kNoAllocMaxNodes / 32;   // bitmap words (4 MiB)
+static __device__ __align__(256) unsigned scratch[kNoAllocWords + 2];
+constexpr int kMaxDevices = 64;
+static void* g_scratch_dev[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Per-device address of `scratch`, looked up once per device with that device
+// current (the library's init call, mpm_create, does it).
+static hipError_t scratch_address(int dev, void** out) {
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return hipSuccess;
+}
+
 struct ImplBase;
 
 struct mpm_ctx {
     mpm_config cfg{};
+    int device = 0;               // the device current at mpm_create
+    void* scratch_ptr = nullptr;  // that device's address of `scratch`
     hipStream_t stream = nullptr;
     std::string err = "no error";
     ImplBase* impl = nullptr;
@@ -145,6 +175,7 @@ struct ImplBase {
     virtual int download(mpm_ctx*, double*, double*, double*, double*, double*, double*,
                          double*, double*, double*, double*) = 0;
     virtual int count_active(mpm_ctx*, int64_t*) = 0;
+    virtual int count_active_noalloc(mpm_ctx*, int64_t*) = 0;
     virtual int set_surrogate(mpm_ctx*, mpm_mlp*, double tol) = 0;
     virtual int surrogate_stats(mpm_ctx*, int64_t*, int64_t*) = 0;
     virtual int return_log(mpm_ctx*, int64_t cap, int64_t stride) = 0;
@@ -900,6 +931,39 @@ struct Impl : ImplBase {
         return lin ? count_impl<3, 2>(c, out) : count_impl<3, 3>(c, out);
     }
 
+    // The same count with the node bitmap and the counter in the file-scope
+    // scratch array: clear (a fill kernel, not hipMemsetAsync, so the calls
+    // stay capturable), mark, count, read back.  No allocation.
+    template <int DIM, int S>
+    int count_noalloc_impl(mpm_ctx* c, int64_t* out) {
+        if (ni > kNoAllocMaxNodes)
+            return c->fail(MPM_E_RANGE, "mpm_count_active_nodes_noalloc: " + std::to_string(ni) +
+                                            " grid nodes exceed its maximum of " +
+                                            std::to_string(kNoAllocMaxNodes));
+        if (!c->scratch_ptr) return c->fail(MPM_E_STATE, "mpm_count_active_nodes_noalloc: no scratch address");
+        unsigned* words = static_cast<unsigned*>(c->scratch_ptr);
+        unsigned long long* cnt = reinterpret_cast<unsigned long long*>(words + kNoAllocWords);
+        const long long nw = (ni + 31) / 32;
+        k_fill_u32<<<unsigned((nw + 255) / 256), 256, 0, c->stream>>>(words, nw);
+        k_fill_u32<<<1, 256, 0, c->stream>>>(words + kNoAllocWords, 2);
+        k_mark_active_bits<DIM, S, R><<<nblocks(np), kBlock, 0, c->stream>>>(geom, pp, words, int(np));
+        k_count_bits<<<1024, 256, 0, c->stream>>>(words, nw, cnt);
+        hipError_t e = hipGetLastError();
+        unsigned long long h = 0;
+        if (e == hipSuccess) e = hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+        hipError_t e2 = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = e2;
+        if (e != hipSuccess) return c->hip_fail(e, "mpm_count_active_nodes_noalloc");
+        *out = (int64_t)h;
+        return MPM_OK;
+    }
+
+    int count_active_noalloc(mpm_ctx* c, int64_t* out) override {
+        const bool lin = c->cfg.basis == MPM_LINEAR;
+        if (dim == 2) return lin ? count_noalloc_impl<2, 2>(c, out) : count_noalloc_impl<2, 3>(c, out);
+        return lin ? count_noalloc_impl<3, 2>(c, out) : count_noalloc_impl<3, 3>(c, out);
+    }
+
     template <typename T>
     int fetch(mpm_ctx* c, const T* src, double* host, const char* what) {
         const int n = int(np);
@@ -995,7 +1059,15 @@ int mpm_create(const mpm_config* cfg, mpm_ctx** out) {
     }
     mpm_ctx* c = new mpm_ctx();
     c->cfg = *cfg;
-
+    {
+        hipError_t e0 = hipGetDevice(&c->device);
+        if (e0 == hipSuccess) e0 = scratch_address(c->device, &c->scratch_ptr);
+        if (e0 != hipSuccess) {
+            g_create_error = std::string("scratch address: ") + hipGetErrorString(e0);
+            delete c;
+            return MPM_E_HIP;
+        }
+    }
     if (c->cfg.dim == 2) {
         c->cfg.nnz = 1;
         c->cfg.z1 = 0.0;
@@ -1245,6 +1317,12 @@ int mpm_count_active_nodes(mpm_ctx* c, int64_t* count) {
     if (!c || !count) return MPM_E_ARG;
     if (!c->have_particles) return c->fail(MPM_E_STATE, "mpm_count_active_nodes: nothing uploaded");
     return c->impl->count_active(c, count);
+}
+
+int mpm_count_active_nodes_noalloc(mpm_ctx* c, int64_t* count) {
+    if (!c || !count) return MPM_E_ARG;
+    if (!c->have_particles) return c->fail(MPM_E_STATE, "mpm_count_active_nodes_noalloc: nothing uploaded");
+    return c->impl->count_active_noalloc(c, count);
 }
 
 int mpm_tpeak(int64_t nbytes, int reps, double* gbps) {

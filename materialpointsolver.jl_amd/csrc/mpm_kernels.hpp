@@ -1450,6 +1450,40 @@ __global__ __launch_bounds__(256) void k_mark_active(GridGeom g, PartPtrs<R> pp,
     });
 }
 
+// k_mark_active on a bitmap: bit n % 32 of word n / 32 set for every node n
+// with a non-zero weight.
+template <int DIM, int S, typename R>
+__global__ __launch_bounds__(256) void k_mark_active_bits(GridGeom g, PartPtrs<R> pp,
+                                                          unsigned* __restrict__ words, int np) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    double x[3];
+    load_x<DIM>(pp.x, i, x);
+    Stencil<DIM, S, R> st;
+    int flag = 0;
+    if (!make_stencil<DIM, S, R>(g, x, st, flag)) return;
+    for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
+        if (N != R(0)) atomicOr(&words[n >> 5], 1u << (n & 31));
+    });
+}
+
+// Set bits of words[0, nw), added to *out.
+__global__ __launch_bounds__(256) void k_count_bits(const unsigned* __restrict__ words, long long nw,
+                                                    unsigned long long* out) {
+    __shared__ unsigned long long part[256];
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    unsigned long long c = 0;
+    for (; i < nw; i += stride) c += __popc(words[i]);
+    part[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && part[0]) atomicAdd(out, part[0]);
+}
+
 __global__ __launch_bounds__(256) void k_count_bytes(const unsigned char* __restrict__ mark,
                                                      long long n, unsigned long long* out) {
     __shared__ unsigned long long part[256];

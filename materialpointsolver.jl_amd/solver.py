@@ -204,9 +204,13 @@ class HIPModel:
             p(out["eps_s"]), p(out["ms"])))
         return out
 
-    def count_active_nodes(self):
+    def count_active_nodes(self, noalloc=False):
+        """Grid nodes with a non-zero weight from some particle.  noalloc=True
+        uses mpm_count_active_nodes_noalloc (no device allocation; grids of at
+        most 2**25 nodes)."""
         out = np.zeros(1, dtype=np.int64)
-        self._check(self.lib.mpm_count_active_nodes(self.ctx, capi.iptr(out)))
+        fn = self.lib.mpm_count_active_nodes_noalloc if noalloc else self.lib.mpm_count_active_nodes
+        self._check(fn(self.ctx, capi.iptr(out)))
         return int(out[0])
 
     def set_option(self, option, value):
