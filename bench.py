@@ -34,8 +34,8 @@ METRIC = "particle-steps/s (3D DP column, %HBM peak); batched & surrogate vs pla
 ALGO_BYTES = {
     "k_p2g": (68, 28),
     "k_grid1": (0, 77),
-    "k_g2p1_p2g2": (64, 36),  # re-map gather (+ the scatter when not split)
-    "k_p2g2": (0, 12),        # re-map momentum scatter (its particle reads are overhead)
+    "k_g2p1_p2g2": (64, 36),  # re-map gather + momentum scatter in one pass (24 + 12 per node)
+    "k_p2g2": (0, 12),        # re-map momentum scatter when split (its particle reads are overhead)
     "k_grid2": (0, 29),
     "k_g2p2": (208, 12),
 }
@@ -96,6 +96,12 @@ def parse():
     ap.add_argument("--slope-h", type=float, default=0.5, help="slope grid spacing (0.5 = 20M particles)")
     ap.add_argument("--slope-k0", type=int, default=400, help="damped K0 steps before the runout")
     ap.add_argument("--slope-steps", type=int, default=200, help="timed runout steps per form")
+    ap.add_argument("--scatter", default=None, choices=["rounds", "owned", "owned_layers"],
+                    help="tile-scatter form (default: the library's)")
+    ap.add_argument("--fused-remap", type=int, default=None, choices=[0, 1],
+                    help="one-pass MUSL re-map (default: the library's)")
+    ap.add_argument("--tile-partials", type=int, default=None, choices=[0, 1],
+                    help="tile partials summed by the grid solves (default: the library's)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -104,13 +110,33 @@ def parse():
 # (a = b + s c over three float4 arrays of TRIAD_BYTES each: reads 2x, writes 1x).
 TRIAD_BYTES = 256 << 20
 KERNEL_RE = {
-    "k_p2g": r"mpm::k_p2g<|mpm::k_scatter_tiled<\d, \d, \d, \w+, 0>",
+    "k_p2g": r"mpm::k_p2g<|mpm::k_scatter_(tiled|owned)<\d, \d, \d, \w+, 0[,>]",
     "k_grid1": r"mpm::k_grid1<",
-    "k_g2p1_p2g2": r"mpm::k_g2p1(_p2g2)?(_tiled)?<",
-    "k_p2g2": r"mpm::k_scatter_tiled<\d, \d, \d, \w+, 1>",
+    "k_g2p1_p2g2": (r"mpm::k_g2p1(_p2g2)?(_tiled)?<|mpm::k_scatter_tiled<\d, \d, \d, \w+, [23]>"
+                    r"|mpm::k_remap_owned<"),
+    "k_p2g2": r"mpm::k_scatter_(tiled|owned)<\d, \d, \d, \w+, 1[,>]",
     "k_grid2": r"mpm::k_grid2<",
     "k_g2p2": r"mpm::k_g2p2(_tiled)?<",
 }
+
+
+def apply_options(a, model):
+    """The kernel-variant options given on the command line."""
+    if a.scatter is not None:
+        model.set_option("scatter", a.scatter)
+    if a.fused_remap is not None:
+        model.set_option("fused_remap", a.fused_remap)
+    if a.tile_partials is not None:
+        model.set_option("tile_partials", a.tile_partials)
+
+
+def variant_args(a):
+    out = []
+    for flag, v in (("--scatter", a.scatter), ("--fused-remap", a.fused_remap),
+                    ("--tile-partials", a.tile_partials)):
+        if v is not None:
+            out += [flag, str(v)]
+    return out
 
 
 def pmc_child(a):
@@ -120,6 +146,7 @@ def pmc_child(a):
     mpm = ge.load_package()
     args, grid, mp, attr, bc = mpm.scenes.column_collapse_3d(h=a.h, eps=a.precision)
     model = mpm.HIPModel(grid, mp, attr, bc, sort_interval=a.sort_interval)
+    apply_options(a, model)
     sargs = model.step_args(args)
     model.run(sargs, float(args.ΔT), 0.0, 14)
     model.sync()
@@ -166,7 +193,8 @@ def measure_traffic(a):
         d = tempfile.mkdtemp(prefix="mpm_pmc_", dir="/tmp")
         cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv",
                "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__), "--pmc-child",
-               "--h", str(a.h), "--precision", a.precision, "--sort-interval", str(a.sort_interval)]
+               "--h", str(a.h), "--precision", a.precision, "--sort-interval", str(a.sort_interval),
+               *variant_args(a)]
         print(f"# traffic: rocprofv3 --pmc {counter}", file=sys.stderr, flush=True)
         r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True)
         files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
@@ -243,6 +271,7 @@ def main():
     args, grid, mp, attr, bc = mpm.scenes.column_collapse_3d(h=a.h, eps=a.precision)
     dt = float(args.ΔT)
     model = mpm.HIPModel(grid, mp, attr, bc, sort_interval=a.sort_interval)
+    apply_options(a, model)
     sargs = model.step_args(args)
     print(f"# setup {time.time() - t_setup:.1f}s: np={mp.np} ni={grid.ni} dt={dt:.4e}",
           file=sys.stderr, flush=True)
@@ -255,10 +284,13 @@ def main():
 
     def table(prof):
         kern = {}
+        split = prof.get("k_p2g2", (0, 0))[1] > 0
         for name, (ms, cnt) in prof.items():
             if cnt == 0:
                 continue
             per_p, per_n = ALGO_BYTES.get(name) or AUX_BYTES.get(name, (0, 0))
+            if name == "k_g2p1_p2g2" and split:
+                per_n -= ALGO_BYTES["k_p2g2"][1]  # the gather alone: vsT, as
             algo = per_p * n + per_n * n_active
             avg_s = ms * 1e-3 / cnt
             kern[name] = dict(avg_us=avg_s * 1e6, launches=cnt, total_ms=ms,
@@ -300,7 +332,9 @@ def main():
         print(f"#   {name:14s} {k['avg_us']:9.1f} us x {k['launches']:5d}  {g}",
               file=sys.stderr, flush=True)
 
-    step_bytes = sum(ALGO_BYTES[k][0] * n + ALGO_BYTES[k][1] * n_active for k in ALGO_BYTES)
+    # k_g2p1_p2g2's entry already holds the re-map scatter's 12 B per node
+    step_bytes = sum(ALGO_BYTES[k][0] * n + ALGO_BYTES[k][1] * n_active for k in ALGO_BYTES
+                     if k != "k_p2g2")
     tuned = None
     if not a.no_autotune:
         best, tab, Ti = mpm.autotune(model, sargs, dt, Ti, steps=40, warm=10,
@@ -353,6 +387,11 @@ def main():
             "grid_nodes": int(grid.ni),
             "active_nodes": n_active,
             "sort_interval": a.sort_interval,
+            "kernel_variant": {"scatter": a.scatter or "rounds (default)",
+                               "fused_remap": bool(a.fused_remap) if a.fused_remap is not None
+                               else "off (default)",
+                               "tile_partials": bool(a.tile_partials) if a.tile_partials is not None
+                               else "on (default)"},
             "step_algorithmic_bytes": step_bytes,
             "step_hbm_fraction": step_bytes / (elapsed / a.steps) / (HBM_PEAK_GBS * 1e9),
             "kernels_us": {k: round(v["avg_us"], 2) for k, v in kernels.items()},
@@ -552,12 +591,9 @@ def surrogate_leg(a, mpm):
     n = int(mp.np)
     plain = mpm.HIPModel(grid, mp, attr, bc, sort_interval=a.sort_interval)
     sa = plain.step_args(args)
-    plain.return_log(2_000_000, 16)
-    t0 = time.perf_counter()
-    plain.run(sa, dt, 0.0, a.sur_steps)
-    plain.sync()
-    X, Y = plain.read_return_log()
-    plain.return_log(0, 1)
+    # samples over the whole span the learned form runs (2 x sur_steps from rest)
+    _, X, Y = plain.run_logged(sa, dt, 0.0, 2 * a.sur_steps, chunk=max(1, a.sur_steps // 4),
+                               capacity=250_000, stride=13)
     if len(X) < 1000:
         plain.close()
         return {"skipped": f"only {len(X)} return-mapping samples in {a.sur_steps} steps"}
@@ -645,9 +681,9 @@ def slope_leg(a, mpm):
         t_k0 = time.perf_counter() - t0
         k0 = m.download()
         ra = m.step_args(lay["runout_args"])
-        m.return_log(2_000_000, 8)
-        m.run(ra, dt, Ti, 60)
-        X, Y = m.read_return_log()
+        # samples over the whole span the timed forms run (20 + slope_steps)
+        _, X, Y = m.run_logged(ra, dt, Ti, 20 + a.slope_steps, chunk=max(1, a.slope_steps // 8),
+                               capacity=1_000_000, stride=29, graph=True)
     finally:
         m.close()
     print(f"# slope: K0 {lay['k0_steps']} steps in {t_k0:.2f}s, {len(X)} return samples",

@@ -169,18 +169,19 @@ int mpm_run_graph(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int6
  *                         (default).  Fewer event pairs per step keep the
  *                         profiled steps closer to unprofiled ones.
  *  MPM_OPT_SCATTER:       form of the tile scatters (P2G and the MUSL re-map):
- *                         MPM_SCATTER_OWNED (default): each lane owns one
- *                         stencil base of a tile and sums its particles in
- *                         registers (3D P2G: mass and momentum, then force);
+ *                         MPM_SCATTER_ROUNDS (default): rank rounds of
+ *                         distinct bases adding into an LDS window;
+ *                         MPM_SCATTER_OWNED: each lane owns one stencil base
+ *                         of a tile and sums its particles in registers (3D
+ *                         P2G: mass and momentum, then force);
  *                         MPM_SCATTER_OWNED_LAYERS: the same, one z layer of
- *                         the stencil per pass (fewer registers, more waves);
- *                         MPM_SCATTER_ROUNDS: rank rounds of distinct bases
- *                         adding into an LDS window.
- *  MPM_OPT_FUSED_REMAP:   1 (default): with a base-owned scatter form, the MUSL
- *                         re-map gathers, moves the particles and sums the new
- *                         momentum in one kernel (not FP64 3D uGIMP, whose sums
- *                         exceed the register file); 0: a gather kernel, then
- *                         the momentum scatter.
+ *                         the stencil per pass (fewer registers, more passes).
+ *  MPM_OPT_FUSED_REMAP:   0 (default): with the tile kernels, the MUSL re-map
+ *                         is a gather kernel, then the momentum scatter;
+ *                         1: it gathers, moves the particles and scatters the
+ *                         new momentum in one kernel (rank rounds: every
+ *                         precision; the base-owned forms: not in FP64 3D
+ *                         uGIMP, whose sums exceed the register file).
  *  MPM_OPT_TILE_PARTIALS: 1 (default): the tile scatters store each tile's
  *                         window sums and the grid solves add the windows that
  *                         hold a node, in a fixed order (no flush atomics);

@@ -26,7 +26,7 @@ SCATTER_FORMS = ("owned", "owned_layers", "rounds")
 
 
 def autotune(model, sargs, dt, Ti, *, sort_intervals=(5, 10, 20, 40), tiles=(True, False),
-             scatters=("owned",), steps=40, warm=10, graph=False, clock=time.perf_counter):
+             scatters=("rounds",), steps=40, warm=10, graph=False, clock=time.perf_counter):
     """Returns (best, table, Ti): best = dict(sort_interval=..., tiles=...,
     scatter=...), table = list of (sort_interval, tiles, scatter, seconds per
     step), Ti = time after the trials.  The scatter form only matters with

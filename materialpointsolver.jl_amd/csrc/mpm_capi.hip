@@ -38,6 +38,8 @@ enum { PH_FULL = 0, PH_HEAD = 1, PH_BODY = 2 };
 constexpr int kBlock = 256;
 
 inline unsigned nblocks(long long n) { return unsigned((n + kBlock - 1) / kBlock); }
+// Blocks for the grid solves (node_block): a multiple of the 8 XCDs.
+inline unsigned nblocks_xcd(long long n) { return (nblocks(n) + 7u) & ~7u; }
 
 // Julia's Base.min for IEEE floats (NaN propagates).
 inline double jl_min(double x, double y) {
@@ -97,8 +99,8 @@ struct mpm_ctx {
     bool prof_on = false;
     uint64_t prof_mask = 0;  // kernel ids to time; 0 = all
     bool use_tiles = true;  // LDS-binned scatters when the particles are tile-sorted
-    int scatter = MPM_SCATTER_OWNED;  // tile scatter form (MPM_OPT_SCATTER)
-    bool fused_remap = true;          // one-pass MUSL re-map (MPM_OPT_FUSED_REMAP)
+    int scatter = MPM_SCATTER_ROUNDS;  // tile scatter form (MPM_OPT_SCATTER)
+    bool fused_remap = false;         // one-pass MUSL re-map (MPM_OPT_FUSED_REMAP)
     bool tile_partials = true;        // tile scatters store windows (MPM_OPT_TILE_PARTIALS)
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> pool;
@@ -395,15 +397,8 @@ struct Impl : ImplBase {
             return c->fail(MPM_E_ARG, "mpm_upload_particles: NULL array");
         int rc;
         hipError_t e;
-        for (int a = 0; a < dim; ++a) {
-            e = hipMemcpy(pp.x[a], xi + a * np, sizeof(double) * np, hipMemcpyHostToDevice);
-            if (e != hipSuccess) return c->hip_fail(e, "upload xi");
-            if ((rc = put(c, pp.v[a], vs + a * np, "upload vs")) != MPM_OK) return rc;
-        }
-        for (int k = 0; k < ns; ++k) {
-            if ((rc = put(c, pp.sig[k], sigma + k * np, "upload sigma")) != MPM_OK) return rc;
-            if ((rc = put(c, pp.eps[k], eps_s + k * np, "upload eps_s")) != MPM_OK) return rc;
-        }
+        // every check runs before the first copy, so a rejected upload leaves
+        // the device state as it was
         std::vector<double> J(np), ms(np);
         std::vector<int> nid32(np), pid(np);
         for (long long i = 0; i < np; ++i) {
@@ -426,6 +421,15 @@ struct Impl : ImplBase {
                   (dim == 2 || std::isfinite(xi[2 * np + i]))))
                 return c->fail(MPM_E_RANGE, "mpm_upload_particles: particle " + std::to_string(i + 1) +
                                                 " has a non-finite position");
+        }
+        for (int a = 0; a < dim; ++a) {
+            e = hipMemcpy(pp.x[a], xi + a * np, sizeof(double) * np, hipMemcpyHostToDevice);
+            if (e != hipSuccess) return c->hip_fail(e, "upload xi");
+            if ((rc = put(c, pp.v[a], vs + a * np, "upload vs")) != MPM_OK) return rc;
+        }
+        for (int k = 0; k < ns; ++k) {
+            if ((rc = put(c, pp.sig[k], sigma + k * np, "upload sigma")) != MPM_OK) return rc;
+            if ((rc = put(c, pp.eps[k], eps_s + k * np, "upload eps_s")) != MPM_OK) return rc;
         }
         e = hipMemcpy(pp.J, J.data(), sizeof(double) * np, hipMemcpyHostToDevice);
         if (e != hipSuccess) return c->hip_fail(e, "upload J");
@@ -564,7 +568,7 @@ struct Impl : ImplBase {
             sa.log_stride = log_stride;
         }
         const int n = int(np);
-        const unsigned pb = nblocks(np), gb = nblocks(ni > nc ? ni : nc);
+        const unsigned pb = nblocks(np), gb = nblocks_xcd(ni > nc ? ni : nc);
         hipStream_t s = c->stream;
         int* fl = c->d_flags;
         int rc;
@@ -578,6 +582,8 @@ struct Impl : ImplBase {
         TilePartials<R> tpart{};
         if (tbp) {
             constexpr int WWt = Win<DIM, S, TSH>::WW;
+            // add_partials reads at most two tile windows per axis
+            static_assert(WWt <= (2 << TSH), "a tile window spans at most two tiles per axis");
             tpart.tb = tbp;
             tpart.slot_of = sort.slot_of;
             tpart.tsh = TSH;
@@ -623,13 +629,24 @@ struct Impl : ImplBase {
                  })) != MPM_OK)
                 return rc;
             // re-map gather (tiled: shared window, 256 threads); MUSL then scatters the
-            // new momentum (tiled: its own kernel, or the one-pass base-owned re-map;
-            // otherwise fused, global atomics).  The one-pass form's FP64 3D uGIMP
-            // sums do not fit the register file: FP64 3D uGIMP keeps the two kernels.
-            const bool remap1 = tiled && owned && sa.remap && c->fused_remap &&
-                                !(DIM == 3 && S == 3 && sizeof(R) == 8);
+            // new momentum (tiled: its own kernel, or one pass with the gather in the
+            // scatter's rounds or in the base-owned sums; otherwise fused, global
+            // atomics).  The base-owned one-pass form's FP64 3D uGIMP sums do not
+            // fit the register file: there FP64 3D uGIMP keeps the two kernels.
+            const bool remap1 = tiled && sa.remap && c->fused_remap &&
+                                (!owned || !(DIM == 3 && S == 3 && sizeof(R) == 8));
             if ((rc = c->launch(KID_G2P1, [&] {
-                     if (remap1) {
+                     if (remap1 && !owned) {
+                         // the rank-round scatter with the gather and move in its rounds
+                         if (sa.va != 0 || sa.pic != R(0))
+                             k_scatter_tiled<DIM, S, TSH, R, 3><<<sort.tiled_grid(), 64, 0, s>>>(
+                                 geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
+                                 sort.seg_begin, sort.seg_end, sort.tile_count, fl, tbp, sa);
+                         else
+                             k_scatter_tiled<DIM, S, TSH, R, 2><<<sort.tiled_grid(), 64, 0, s>>>(
+                                 geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
+                                 sort.seg_begin, sort.seg_end, sort.tile_count, fl, tbp, sa);
+                     } else if (remap1) {
                          if constexpr (!(DIM == 3 && S == 3 && sizeof(R) == 8)) {
                              if (sa.va != 0 || sa.pic != R(0))
                                  k_remap_owned<DIM, S, TSH, R, true><<<sort.tiled_grid(), 64, 0, s>>>(
@@ -668,7 +685,7 @@ struct Impl : ImplBase {
             }
             if (sa.remap) {
                 if ((rc = c->launch(KID_GRID2, [&] {
-                         k_grid2<DIM, R><<<nblocks(ni), kBlock, 0, s>>>(geom, gp, R(dt), tpart);
+                         k_grid2<DIM, R><<<nblocks_xcd(ni), kBlock, 0, s>>>(geom, gp, R(dt), tpart);
                      })) != MPM_OK)
                     return rc;
             }

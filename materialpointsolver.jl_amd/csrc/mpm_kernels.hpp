@@ -163,8 +163,22 @@ struct TilePartials {
     int tn[3];  // tiles per axis
 };
 
+// Node block of a grid-solve workgroup.  Consecutive workgroups are dispatched
+// to the 8 XCDs in turn; block b is taken as block (b mod 8) per + b / 8 (per =
+// gridDim.x / 8, the grid launched as a multiple of 8 blocks), so each XCD
+// sweeps one contiguous range of nodes (a slab of z layers) and the tile
+// windows that range reads are fetched into that XCD's L2 only.
+__device__ __forceinline__ long long node_block() {
+    const unsigned b = blockIdx.x, per = gridDim.x >> 3;
+    return (long long)((b & 7u) * per + (b >> 3));
+}
+
 // Adds to acc[0, NF) the partials of node n from every tile window that holds
-// it, in a fixed order (tiles with lower indices first).
+// it, in a fixed order (tiles with lower indices first).  A window spans
+// ww <= 2^(tsh+1) nodes per axis, so at most two tiles per axis hold a node
+// (2^DIM candidates): every candidate's slot is loaded first and then every
+// held window's values, so a node waits for two memory round trips rather
+// than two per candidate.
 template <int DIM, int NF, typename R>
 __device__ __forceinline__ void add_partials(const GridGeom& g, const TilePartials<R>& tp,
                                              long long n, R* acc) {
@@ -195,17 +209,33 @@ __device__ __forceinline__ void add_partials(const GridGeom& g, const TilePartia
             lo[a] = hi[a] = 0;
         }
     }
-    for (int tz = lo[2]; tz <= hi[2]; ++tz)
-        for (int ty = lo[1]; ty <= hi[1]; ++ty)
-            for (int tx = lo[0]; tx <= hi[0]; ++tx) {
-                const int s = tp.slot_of[tx + tp.tn[0] * (ty + tp.tn[1] * tz)];
-                if (s < 0) continue;
-                const int k = (i[0] - (tx << tp.tsh)) +
-                              tp.ww * ((i[1] - (ty << tp.tsh)) + (DIM == 3 ? tp.ww * (i[2] - (tz << tp.tsh)) : 0));
-                const R* b = tp.tb + (long long)s * NF * tp.wn + k;
+    constexpr int NC = 1 << DIM;
+    int koff[NC];
+    int slot[NC];
+    // candidate c = (dz, dy, dx) bits: tile hi - d; c = NC - 1 is the lowest tile
 #pragma unroll
-                for (int f = 0; f < NF; ++f) acc[f] += b[f * tp.wn];
-            }
+    for (int c = 0; c < NC; ++c) {
+        const int dx = c & 1, dy = (c >> 1) & 1, dz = DIM == 3 ? (c >> 2) & 1 : 0;
+        const int tx = hi[0] - dx, ty = hi[1] - dy, tz = hi[2] - dz;
+        const bool ok = tx >= lo[0] && ty >= lo[1] && tz >= lo[2];
+        slot[c] = ok ? tp.slot_of[tx + tp.tn[0] * (ty + tp.tn[1] * tz)] : -1;
+        koff[c] = (i[0] - (tx << tp.tsh)) +
+                  tp.ww * ((i[1] - (ty << tp.tsh)) + (DIM == 3 ? tp.ww * (i[2] - (tz << tp.tsh)) : 0));
+    }
+    R v[NC][NF];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const R* b = tp.tb + (long long)(slot[c] < 0 ? 0 : slot[c]) * NF * tp.wn + koff[c];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) v[c][f] = slot[c] < 0 ? R(0) : b[f * tp.wn];
+    }
+    // lower tile indices first: tz, then ty, then tx ascending
+#pragma unroll
+    for (int c = NC - 1; c >= 0; --c)
+        if (slot[c] >= 0) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) acc[f] += v[c][f];
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -478,7 +508,7 @@ __global__ __launch_bounds__(256) void k_p2g(GridGeom g, PartPtrs<R> pp, GridPtr
 template <int DIM, typename R>
 __global__ __launch_bounds__(256) void k_grid1(GridGeom g, GridPtrs<R> gp, StepArgs<R> sa,
                                                int ncells, TilePartials<R> tp) {
-    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = node_block() * blockDim.x + threadIdx.x;
     if (n < ncells && sa.mvl) {
         gp.cell_sm[n] = R(0);
         gp.cell_vol[n] = R(0);
@@ -655,7 +685,7 @@ __global__ __launch_bounds__(256) void k_g2p1_p2g2(GridGeom g, PartPtrs<R> pp, G
 // ---------------------------------------------------------------------------
 template <int DIM, typename R>
 __global__ __launch_bounds__(256) void k_grid2(GridGeom g, GridPtrs<R> gp, R dt, TilePartials<R> tp) {
-    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = node_block() * blockDim.x + threadIdx.x;
     if (n >= g.ni) return;
     const R m = gp.m[n];
     if (m == R(0)) return;  // no mass: p is zero and no G2P weight reads du here

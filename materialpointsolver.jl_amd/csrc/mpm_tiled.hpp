@@ -314,20 +314,39 @@ __device__ __forceinline__ void p2g3_window_paired(const Stencil<3, S, float>& s
 // momentum and internal + external force.  MODE 1: the momentum scatter of the
 // MUSL re-map (doublemapping2_OS!, utils_OS.jl:942-959) from the new particle
 // momentum that k_g2p1_tiled stored in pp.pm, with the step-start basis.
+// MODE 2 / 3: the whole MUSL re-map (doublemapping1(_a)_OS! +
+// doublemapping2_OS!, utils_OS.jl:698-959) in the rounds: the tile's vsT, as
+// (MODE 3: and vs, see pic_field) are staged in LDS, each particle gathers
+// from there (from the grid outside the window), moves (v and the deferred
+// position increment), and scatters its new momentum in the same round with
+// the same step-start basis; the position is read once and the momentum never
+// goes through memory.
 // ---------------------------------------------------------------------------
 template <int DIM, int S, int TSH, typename R, int MODE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_scatter_tiled(
     GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, Gravity<R> G,
     const unsigned* __restrict__ keys, const int* __restrict__ tile_list,
     const int* __restrict__ seg_begin, const int* __restrict__ seg_end,
-    const int* __restrict__ count_ptr, int* flags, R* __restrict__ tb) {
+    const int* __restrict__ count_ptr, int* flags, R* __restrict__ tb, StepArgs<R> sa = {}) {
     using W = Win<DIM, S, TSH>;
     constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
     constexpr int NF = MODE == 0 ? 1 + 2 * DIM : DIM;
+    constexpr bool REMAP = MODE >= 2;           // gather + move + scatter
+    constexpr bool VS = MODE == 3;              // the re-map gathers vs as well
+    constexpr int NG = (VS ? 3 : 2) * DIM;      // gathered fields
     constexpr int NWIN = kScatterWindows;
     static_assert(NWIN == 1 || NWIN == 2, "one or two scatter windows");
     __shared__ __attribute__((aligned(16))) R win_[NWIN * NF * WN];
     __shared__ unsigned claim_[NWIN * W::NBN];
+    __shared__ R gwin[REMAP ? NG * WN : 1];
+    R* const* vsf = pic_field(gp, sa);
+    R* gfields[3 * DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        gfields[d] = gp.vsT[d];
+        gfields[DIM + d] = gp.as[d];
+        gfields[2 * DIM + d] = vsf[d];
+    }
     lds_ptr<R> win = (lds_ptr<R>)win_;
     lds_ptr<unsigned> claim = (lds_ptr<unsigned>)claim_;
     R* fields[NF];
@@ -352,6 +371,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         int w0[3];
         tile_window_origin<DIM>(tg, tile, w0);
         for (int k = threadIdx.x; k < NWIN * NF * WN; k += 64) win[k] = R(0);
+        if constexpr (REMAP) {
+            __syncthreads();  // the previous tile's gathers from gwin are done
+            stage_window<DIM, WW, ZS, WN, NG, R>(g, w0, gfields, gwin);
+            __syncthreads();
+        }
         // Rounds are software-pipelined: the keys of the next round come from a
         // two-register sliding window over the sorted keys (a round has at most
         // 64 particles, so the next round lies in [cur, cur + 128)), and the next
@@ -394,6 +418,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     P.sg[4] = pp.sig[4][i];
                     P.sg[5] = pp.sig[5][i];
                 }
+            } else if constexpr (REMAP) {
+                P.m = pp.ms[i];
+#pragma unroll
+                for (int a = 0; a < DIM; ++a) P.v[a] = pp.v[a][i];
             } else {
 #pragma unroll
                 for (int a = 0; a < DIM; ++a) P.v[a] = pp.pm[a][i];
@@ -422,11 +450,51 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const int i = cur + lane;
                 Stencil<DIM, S, R> st;
                 int fl2 = 0;
-                if (make_stencil<DIM, S, R>(g, P.x, st, MODE == 0 ? flag : fl2)) {
+                const bool ok = make_stencil<DIM, S, R>(g, P.x, st, MODE == 0 ? flag : fl2);
+                if (REMAP && !ok) {
+#pragma unroll
+                    for (int a = 0; a < DIM; ++a) pp.dxp[a][i] = R(0);  // frozen particle
+                }
+                if (ok) {
                     R pv[3] = {0, 0, 0};
                     const R m = P.m, vol = P.vol;
+                    if constexpr (REMAP) {
+                        // the re-map gather with the step-start basis, then the move
+                        int gb, gidx;
+                        const bool gin = window_slot<DIM, NB, WW, ZS>(st.base, w0, gb, gidx);
+                        R A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, C[3] = {0, 0, 0};
+                        if (gin) {
+                            // branch-free: zero weights add exact zeros (staged values are finite)
+                            for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R, R, R) {
+                                const R* L = gwin + gb + off;
+                                R vT[3], vA[3], vS[3];
 #pragma unroll
-                    for (int a = 0; a < DIM; ++a) pv[a] = MODE == 0 ? m * P.v[a] : P.v[a];
+                                for (int d = 0; d < DIM; ++d) {
+                                    vT[d] = L[d * WN];
+                                    vA[d] = L[(DIM + d) * WN];
+                                    vS[d] = VS ? L[(2 * DIM + d) * WN] : R(0);
+                                }
+                                g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
+                            });
+                        } else {
+                            for_stencil<DIM, S, R>(g, st, [&](long long n, R N, R, R, R) {
+                                if (N != R(0)) {
+                                    R vT[3], vA[3], vS[3];
+#pragma unroll
+                                    for (int d = 0; d < DIM; ++d) {
+                                        vT[d] = gp.vsT[d][n];
+                                        vA[d] = gp.as[d][n];
+                                        vS[d] = VS ? vsf[d][n] : R(0);
+                                    }
+                                    g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
+                                }
+                            });
+                        }
+                        g2p1_commit<DIM, R>(pp, sa, i, A, B, C, pv, m, P.v);
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < DIM; ++a) pv[a] = MODE == 0 ? m * P.v[a] : P.v[a];
+                    }
                     auto contrib = [&](R N, R dNx, R dNy, R dNz, R* c) {
                         if constexpr (MODE == 0) {
                             p2g_values<DIM, R>(N, dNx, dNy, dNz, m, vol, pv, P.sg, G, c);

@@ -216,7 +216,7 @@ class HIPModel:
     def set_option(self, option, value):
         """Runtime option: "sort_interval" (>= 1), "tiles" (bool), "scatter"
         (the tile scatter form: "owned", "owned_layers" or "rounds"),
-        "fused_remap" (bool: the one-pass MUSL re-map with the owned forms),
+        "fused_remap" (bool: the one-pass MUSL re-map; off by default),
         "tile_partials" (bool: tile windows stored and summed by the grid solves
         instead of flushed with atomics) or
         "profile_kernels" (kernel names that profile_enable times; empty or
@@ -258,6 +258,28 @@ class HIPModel:
             self._check(self.lib.mpm_return_log_read(self.ctx, capi.fptr(X), capi.fptr(Y), n,
                                                      capi.iptr(rows)))
         return X, Y
+
+    def run_logged(self, sargs, dt, Ti, nsteps, chunk=200, capacity=2_000_000, stride=1,
+                   graph=False):
+        """Run nsteps with the exact return, logging return-mapping samples in
+        chunks of `chunk` steps: each chunk logs up to `capacity` samples (every
+        stride-th yielding particle in the current particle order, until the
+        chunk's log is full), so the samples cover the whole run rather than its
+        first steps.  Returns (Ti, X, Y) with X (n, 16) and Y (n, 7) float32."""
+        xs, ys = [], []
+        done = 0
+        while done < nsteps:
+            k = min(chunk, nsteps - done)
+            self.return_log(capacity, stride)
+            Ti = self.run(sargs, dt, Ti, k, graph=graph)
+            X, Y = self.read_return_log()
+            xs.append(X)
+            ys.append(Y)
+            done += k
+        self.return_log(0, 1)
+        X = np.concatenate(xs) if xs else np.zeros((0, 16), np.float32)
+        Y = np.concatenate(ys) if ys else np.zeros((0, 7), np.float32)
+        return Ti, X, Y
 
     def set_surrogate(self, surrogate, tol=1e-3):
         """Use a trained Surrogate as the 3D Mohr-Coulomb return (None detaches)."""

@@ -132,9 +132,12 @@ class Surrogate:
         return p
 
     def fit_normalisation(self, X, Y):
-        """Standardise inputs and scale outputs from a sample set (keeps weights)."""
-        X = np.asarray(X, dtype=np.float64)
-        Y = np.asarray(Y, dtype=np.float64)
+        """Standardise inputs and scale outputs from a sample set (keeps weights).
+        Sets of more than 4M rows are summarised from an evenly strided subset
+        of about 4M rows."""
+        step = max(1, len(X) // 4_000_000)
+        X = np.asarray(X[::step], dtype=np.float64)
+        Y = np.asarray(Y[::step], dtype=np.float64)
         p = self.params()
         mean = X.mean(0)
         std = X.std(0)

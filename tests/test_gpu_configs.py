@@ -177,3 +177,90 @@ def test_slope_20m_k0_and_runout(mpm):
     print(f"runout after 2000 steps ({2000 * dt:.2f} s): mean dx {dx.mean():.3f} m, max {dx.max():.2f} m",
           flush=True)
     assert dx.mean() > 0.0  # the failing mass moves downslope
+
+
+def test_slope_20m_learned_return_vs_exact(mpm):
+    # The learned 3D Mohr-Coulomb return on the full slope.  K0 once; the plain
+    # (exact-return) runout from the K0 state logs return-mapping samples over
+    # its whole length (50M at most); the network is trained on them on the
+    # GPU; then the same runout is run from the K0 state in fresh contexts with
+    # the exact and with the learned return.  The learned return takes the
+    # branch the network scores highest where that branch's conditions hold and
+    # the yield residual is within 1e-3, and falls back to the exact return
+    # elsewhere.
+    args, grid, mp, attr, bc, lay = mpm.scenes.slope_3d()
+    print(f"case: slope learned return, {mp.np} particles", flush=True)
+    x0 = np.asarray(mp.ξ, dtype=np.float64)
+    dt, nrun = lay["dt"], 2000
+    ra = lay["runout_args"]
+    m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=5)
+    try:
+        Ti = m.run(m.step_args(args), dt, 0.0, lay["k0_steps"], graph=True)
+        s0 = m.download()
+        _, X, Y = m.run_logged(m.step_args(ra), dt, Ti, nrun, chunk=200, capacity=5_000_000,
+                               stride=29, graph=True)
+    finally:
+        m.close()
+    print(f"{len(X)} return samples logged over {nrun} runout steps; branch frequencies "
+          f"(main, edge 12, edge 23, apex) {np.round(Y[:, :4].mean(0), 4)}", flush=True)
+    assert len(X) >= 1_000_000
+    group = dict(coords=s0["xi"], velocity_s=s0["vs"], stress=s0["sigma"], strain_s=s0["eps_s"],
+                 eqstrain=s0["eps_q"], ekstrain=s0["eps_k"], volume=s0["omega"], mass_s=s0["ms"],
+                 time=np.asarray(Ti))
+    S = mpm.Surrogate(batch_capacity=8192, seed=0)
+    try:
+        S.fit_normalisation(X, Y)
+        steps = max(2000, min(8000, len(X) // 8192))  # about one pass over the samples
+        hist = S.train(X, Y, steps=steps, batch=8192, lr=2e-3, seed=5)
+        print(f"trained {steps} steps of 8192: loss {hist[:20].mean():.4f} -> "
+              f"{hist[-20:].mean():.5f}", flush=True)
+        del X, Y
+        out = {}
+        for form in ("exact", "learned"):
+            r = mpm.HIPModel(grid, mp, attr, bc, sort_interval=5)
+            try:
+                T0 = r.upload_snapshot(group, mp.Ω0)
+                if form == "learned":
+                    r.set_surrogate(S, tol=1e-3)
+                r.run(r.step_args(ra), dt, T0, nrun, graph=True)
+                out[form] = r.download()
+                if form == "learned":
+                    plastic, fb = r.surrogate_stats()
+            finally:
+                r.close()
+    finally:
+        S.close()
+    xe, xl = out["exact"]["xi"], out["learned"]["xi"]
+    assert np.all(np.isfinite(xl))
+    fr = fb / max(plastic, 1)
+    # runout: advance of the mass front (99.9th percentile of downslope position)
+    # and mean downslope displacement, learned against exact
+    f0 = np.quantile(x0[:, 0], 0.999)
+    adv = (np.quantile(xe[:, 0], 0.999) - f0, np.quantile(xl[:, 0], 0.999) - f0)
+    mdx = ((xe[:, 0] - x0[:, 0]).mean(), (xl[:, 0] - x0[:, 0]).mean())
+    # deposit: thickness map of the mass (mass per 2 m x 2 m column of the
+    # slope frame / (rho area)), compared in L1 over the map
+    ms = out["exact"]["ms"]
+    lo = np.minimum(xe[:, :2].min(0), xl[:, :2].min(0))
+    hi = np.maximum(xe[:, :2].max(0), xl[:, :2].max(0))
+    te, tl = _thickness_slope(xe, ms, lo, hi), _thickness_slope(xl, ms, lo, hi)
+    ddep = np.abs(tl - te).sum() / te.sum()
+    print(f"learned return: {plastic} yielding particle-steps, fallback {fr:.3f}; front advance "
+          f"exact {adv[0]:.4f} m, learned {adv[1]:.4f} m; mean downslope displacement "
+          f"{mdx[0]:.4f} / {mdx[1]:.4f} m; deposit thickness map L1 difference {ddep:.2e}",
+          flush=True)
+    assert plastic > 0 and fr < 0.5
+    assert adv[0] > 0.0 and mdx[0] > 0.0
+    assert abs(adv[1] - adv[0]) <= 0.005 * adv[0]
+    assert abs(mdx[1] - mdx[0]) <= 0.005 * mdx[0]
+    assert ddep <= 0.005
+
+
+def _thickness_slope(x, ms, lo, hi, b=2.0, rho=2000.0):
+    """Deposit thickness map: the mass in each b x b column (x, y of the slope
+    frame) over rho b^2."""
+    n = ((hi - lo) / b).astype(np.int64) + 1
+    ix = ((x[:, 0] - lo[0]) / b).astype(np.int64)
+    iy = ((x[:, 1] - lo[1]) / b).astype(np.int64)
+    m = np.bincount(ix * n[1] + iy, weights=ms, minlength=n[0] * n[1])
+    return m / (rho * b * b)

@@ -123,20 +123,26 @@ def test_tile_partials_and_atomic_flush(mpm, orc, scatter, partials):
              scatter=scatter, tile_partials=partials)
 
 
+@pytest.mark.parametrize("scatter", ["rounds", "owned"])
 @pytest.mark.parametrize("fused", [True, False])
-def test_remap_one_pass_and_split(mpm, orc, fused):
-    # the one-pass base-owned re-map (FP32, and FP64 2D) against the gather +
-    # momentum-scatter pair, both against the oracle; sort interval 400 adds
-    # drifted particles
+def test_remap_one_pass_and_split(mpm, orc, fused, scatter):
+    # the one-pass MUSL re-map (gather, move and momentum scatter in one tile
+    # kernel: in the rank rounds, or base-owned) against the gather +
+    # momentum-scatter pair, both against the oracle; FP64 in 2D and 3D, FP32
+    # in 3D; sort interval 400 adds drifted particles
     tol = dict(xi=2e-4, vs=5e-4, sigma=2e-3, eps_s=5e-4, eps_q=2e-3, omega=1e-5)
+    kw = dict(fused_remap=fused, scatter=scatter)
     for sort in (5, 400):
         _compare(mpm, orc, _c2(mpm, constitutive="druckerprager", cohesion=200.0), 200, 1e-9,
-                 sort_interval=sort, fused_remap=fused)
+                 sort_interval=sort, **kw)
         _compare(mpm, orc, _c2(mpm, constitutive="druckerprager", cohesion=200.0, va="v"), 200,
-                 1e-9, sort_interval=sort, fused_remap=fused)
+                 1e-9, sort_interval=sort, **kw)
         _compare(mpm, orc, _c3(mpm, eps="FP32", constitutive="druckerprager"), 100, tol,
-                 sort_interval=sort, fused_remap=fused)
-    _compare(mpm, orc, _c3(mpm, basis="linear"), 60, 1e-9, sort_interval=5, fused_remap=fused)
+                 sort_interval=sort, **kw)
+        _compare(mpm, orc, _c3(mpm, constitutive="druckerprager", va="v"), 100, 1e-9,
+                 sort_interval=sort, **kw)
+    _compare(mpm, orc, _c3(mpm, basis="linear"), 60, 1e-9, sort_interval=5, **kw)
+    _compare(mpm, orc, _c3(mpm, basis="linear", scheme="USL"), 60, 1e-9, sort_interval=5, **kw)
 
 
 @pytest.mark.parametrize("scatter", ["owned", "owned_layers", "rounds"])

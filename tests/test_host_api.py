@@ -140,9 +140,9 @@ def test_autotune_picks_the_fastest_and_leaves_it_set(mpm):
 
     m = FakeModel()
     best, table, Ti = mpm.autotune(m, None, 0.5, 0.0, steps=4, warm=1, clock=lambda: m.now)
-    assert best == dict(sort_interval=10, tiles=True, scatter="owned")
-    assert m.opts == {"sort_interval": 10, "tiles": True, "scatter": "owned"}
-    assert len(table) == 8 and min(t[3] for t in table) == 1.0
+    assert best == dict(sort_interval=10, tiles=True, scatter="rounds")
+    assert m.opts == {"sort_interval": 10, "tiles": True, "scatter": "rounds"}
+    assert len(table) == 8 and abs(min(t[3] for t in table) - 1.3) < 1e-12
     assert Ti == 8 * 5 * 0.5  # every trial advanced the clock
     # with every scatter form: 3 forms x 4 intervals with tiles + 4 without
     m = FakeModel()
