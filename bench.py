@@ -96,7 +96,8 @@ def parse():
     ap.add_argument("--slope-h", type=float, default=0.5, help="slope grid spacing (0.5 = 20M particles)")
     ap.add_argument("--slope-k0", type=int, default=400, help="damped K0 steps before the runout")
     ap.add_argument("--slope-steps", type=int, default=200, help="timed runout steps per form")
-    ap.add_argument("--scatter", default=None, choices=["rounds", "owned", "owned_layers"],
+    ap.add_argument("--scatter", default=None,
+                    choices=["rounds", "rounds_layered", "owned", "owned_layers"],
                     help="tile-scatter form (default: the library's)")
     ap.add_argument("--fused-remap", type=int, default=None, choices=[0, 1],
                     help="one-pass MUSL re-map (default: the library's)")

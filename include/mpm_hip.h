@@ -175,7 +175,12 @@ int mpm_run_graph(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int6
  *                         of a tile and sums its particles in registers (3D
  *                         P2G: mass and momentum, then force);
  *                         MPM_SCATTER_OWNED_LAYERS: the same, one z layer of
- *                         the stencil per pass (fewer registers, more passes).
+ *                         the stencil per pass (fewer registers, more passes);
+ *                         MPM_SCATTER_ROUNDS_LAYERED: rank rounds of one
+ *                         group adding into three LDS windows, one per
+ *                         stencil z offset, so each round's LDS updates form
+ *                         three independent chains (FP32 3D uGIMP; elsewhere
+ *                         it runs as MPM_SCATTER_ROUNDS).
  *  MPM_OPT_FUSED_REMAP:   0 (default): with the tile kernels, the MUSL re-map
  *                         is a gather kernel, then the momentum scatter;
  *                         1: it gathers, moves the particles and scatters the
@@ -188,7 +193,8 @@ int mpm_run_graph(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int6
  *                         0: the windows are flushed with global atomics. */
 enum { MPM_OPT_SORT_INTERVAL = 0, MPM_OPT_TILES = 1, MPM_OPT_PROFILE_MASK = 2, MPM_OPT_SCATTER = 3,
        MPM_OPT_FUSED_REMAP = 4, MPM_OPT_TILE_PARTIALS = 5 };
-enum { MPM_SCATTER_ROUNDS = 0, MPM_SCATTER_OWNED = 1, MPM_SCATTER_OWNED_LAYERS = 2 };
+enum { MPM_SCATTER_ROUNDS = 0, MPM_SCATTER_OWNED = 1, MPM_SCATTER_OWNED_LAYERS = 2,
+       MPM_SCATTER_ROUNDS_LAYERED = 3 };
 int mpm_set_option(mpm_ctx *ctx, int option, int64_t value);
 
 /* Wait for all work on the context's stream; reports range/finite flags the

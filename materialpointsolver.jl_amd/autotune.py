@@ -8,10 +8,13 @@ re-map):
 * ``"owned_layers"``: the same with one z layer of the stencil per pass (six
   passes, ~130 VGPRs, three wavefronts per SIMD; more particle reloads);
 * ``"rounds"``: rank rounds of distinct bases adding into two LDS windows per
-  wavefront (12.6 KB of LDS per wavefront).
+  wavefront (12.6 KB of LDS per wavefront);
+* ``"rounds_layered"``: rank rounds of one group adding into three layer
+  windows, one per stencil z offset (the same 12.6 KB): three independent
+  chains of LDS updates per round (FP32 3D uGIMP; elsewhere ``"rounds"``).
 
-The tile shape (4x4x4 bases in 3D, 8x8 in 2D) and the windows (no slack) are
-compile-time.
+The tile shape (4x4x4 bases in 3D, 8x8 in 2D) and the window size (no slack)
+are compile-time.
 
 ``autotune`` times candidate settings on the live model, in place: each
 candidate runs ``warm`` untimed steps and ``steps`` timed steps of the actual
@@ -22,7 +25,7 @@ from __future__ import annotations
 
 import time
 
-SCATTER_FORMS = ("owned", "owned_layers", "rounds")
+SCATTER_FORMS = ("owned", "owned_layers", "rounds", "rounds_layered")
 
 
 def autotune(model, sargs, dt, Ti, *, sort_intervals=(5, 10, 20, 40), tiles=(True, False),

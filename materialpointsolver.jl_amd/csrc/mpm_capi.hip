@@ -38,8 +38,6 @@ enum { PH_FULL = 0, PH_HEAD = 1, PH_BODY = 2 };
 constexpr int kBlock = 256;
 
 inline unsigned nblocks(long long n) { return unsigned((n + kBlock - 1) / kBlock); }
-// Blocks for the grid solves (node_block): a multiple of the 8 XCDs.
-inline unsigned nblocks_xcd(long long n) { return (nblocks(n) + 7u) & ~7u; }
 
 // Julia's Base.min for IEEE floats (NaN propagates).
 inline double jl_min(double x, double y) {
@@ -240,7 +238,7 @@ struct Impl : ImplBase {
     GridPtrs<R> gp{};
     Material<R>* d_mats = nullptr;
     double* d_stage = nullptr;  // np doubles, download staging
-    R* tile_part = nullptr;     // tile partials: ntiles x (1 + 2 dim) x window nodes
+    R* tile_part = nullptr;     // tile partials: ntiles x window nodes x 8 (padded fields)
     std::vector<void*> owned;
     SortState<R> sort;
     // learned 3D Mohr-Coulomb return (FP32 only)
@@ -371,11 +369,13 @@ struct Impl : ImplBase {
                 }
             });
             if (ea != hipSuccess) return c->fail(MPM_E_NOMEM, std::string("sort buffers: ") + hipGetErrorString(ea));
-            // tile partials of the widest scatter (P2G #1: 1 + 2 dim fields)
+            // tile partials of the widest scatter (P2G #1: 1 + 2 dim fields,
+            // padded to part_stride values per node)
             const int S = c->cfg.basis == MPM_LINEAR ? 2 : 3;
             const long long ww = (1ll << sort.tg.tsh) + S - 1;
             const long long wn = dim == 3 ? ww * ww * ww : ww * ww;
-            const long long cnt = (long long)sort.tg.ntiles * (1 + 2 * dim) * wn;
+            const long long ns = dim == 3 ? part_stride<7>() : part_stride<5>();
+            const long long cnt = (long long)sort.tg.ntiles * ns * wn;
             if ((rc = alloc(c, &tile_part, cnt, "tile partials")) != MPM_OK) return rc;
         }
         return MPM_OK;
@@ -568,13 +568,17 @@ struct Impl : ImplBase {
             sa.log_stride = log_stride;
         }
         const int n = int(np);
-        const unsigned pb = nblocks(np), gb = nblocks_xcd(ni > nc ? ni : nc);
+        const unsigned pb = nblocks(np), gb = nblocks(ni > nc ? ni : nc);
         hipStream_t s = c->stream;
         int* fl = c->d_flags;
         int rc;
         (void)cf;
         const bool tiled = sort.enabled && sort.tiles_valid && c->use_tiles;
-        const bool owned = c->scatter != MPM_SCATTER_ROUNDS;
+        const bool owned = c->scatter == MPM_SCATTER_OWNED || c->scatter == MPM_SCATTER_OWNED_LAYERS;
+        // layer windows for the rank rounds: FP32 3D uGIMP (elsewhere the
+        // two-window rounds run)
+        constexpr bool kLayerable = DIM == 3 && S == 3 && sizeof(R) == 4;
+        const bool layered = kLayerable && c->scatter == MPM_SCATTER_ROUNDS_LAYERED;
         constexpr int TSH = DIM == 3 ? 2 : 3;
         // tile partials: the tile scatters store their windows, the grid solves
         // add them up (no flush atomics; MPM_OPT_TILE_PARTIALS)
@@ -616,7 +620,12 @@ struct Impl : ImplBase {
             if ((rc = c->launch(KID_P2G, [&] {
                      if (tiled && owned)
                          owned_scatter(std::integral_constant<int, 0>{});
-                     else if (tiled)
+                     else if (tiled && layered) {
+                         if constexpr (kLayerable)
+                             k_scatter_tiled<DIM, S, TSH, R, 0, true><<<sort.tiled_grid(), 64, 0, s>>>(
+                                 geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
+                                 sort.seg_begin, sort.seg_end, sort.tile_count, fl, tbp);
+                     } else if (tiled)
                          k_scatter_tiled<DIM, S, TSH, R, 0><<<sort.tiled_grid(), 64, 0, s>>>(
                              geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
                              sort.seg_begin, sort.seg_end, sort.tile_count, fl, tbp);
@@ -674,18 +683,24 @@ struct Impl : ImplBase {
                 return rc;
             if (tiled && sa.remap && !remap1) {
                 if ((rc = c->launch(KID_P2G2, [&] {
-                         if (owned)
+                         if (owned) {
                              owned_scatter(std::integral_constant<int, 1>{});
-                         else
+                         } else if (layered) {
+                             if constexpr (kLayerable)
+                                 k_scatter_tiled<DIM, S, TSH, R, 1, true><<<sort.tiled_grid(), 64, 0, s>>>(
+                                     geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
+                                     sort.seg_begin, sort.seg_end, sort.tile_count, fl, tbp);
+                         } else {
                              k_scatter_tiled<DIM, S, TSH, R, 1><<<sort.tiled_grid(), 64, 0, s>>>(
                                  geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
                                  sort.seg_begin, sort.seg_end, sort.tile_count, fl, tbp);
+                         }
                      })) != MPM_OK)
                     return rc;
             }
             if (sa.remap) {
                 if ((rc = c->launch(KID_GRID2, [&] {
-                         k_grid2<DIM, R><<<nblocks_xcd(ni), kBlock, 0, s>>>(geom, gp, R(dt), tpart);
+                         k_grid2<DIM, R><<<nblocks(ni), kBlock, 0, s>>>(geom, gp, R(dt), tpart);
                      })) != MPM_OK)
                     return rc;
             }
@@ -1294,9 +1309,11 @@ int mpm_set_option(mpm_ctx* c, int option, int64_t value) {
     } else if (option == MPM_OPT_TILE_PARTIALS) {
         c->tile_partials = value != 0;
     } else if (option == MPM_OPT_SCATTER) {
-        if (value != MPM_SCATTER_ROUNDS && value != MPM_SCATTER_OWNED && value != MPM_SCATTER_OWNED_LAYERS)
+        if (value != MPM_SCATTER_ROUNDS && value != MPM_SCATTER_OWNED && value != MPM_SCATTER_OWNED_LAYERS &&
+            value != MPM_SCATTER_ROUNDS_LAYERED)
             return c->fail(MPM_E_ARG, "mpm_set_option: scatter form is MPM_SCATTER_ROUNDS, "
-                                      "MPM_SCATTER_OWNED or MPM_SCATTER_OWNED_LAYERS");
+                                      "MPM_SCATTER_OWNED, MPM_SCATTER_OWNED_LAYERS or "
+                                      "MPM_SCATTER_ROUNDS_LAYERED");
         c->scatter = int(value);
     } else if (option == MPM_OPT_PROFILE_MASK) {
         if (value < 0 || (kNumKernels < 63 && value >= (int64_t(1) << kNumKernels)))

@@ -30,6 +30,8 @@
 // read or write outside the grid arrays.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 #include <stdint.h>
 
 #include <utility>
@@ -152,9 +154,65 @@ struct StepArgs {
 };
 
 // Tile partials of a scatter (the tile kernels' windows stored per tile, see
-// flush_window): tb[slot][f][k] with the tile's slot from slot_of (-1: no
-// particles), windows of ww nodes per axis starting at tile * 2^tsh.  tb ==
-// nullptr: the scatter added atomically into the grid fields.
+// flush_window): node-major, tb[slot][k][f] with the NF fields of window node
+// k padded to part_stride<NF>() values (one or two vector accesses per node),
+// the tile's slot from slot_of (-1: no particles), windows of ww nodes per
+// axis starting at tile * 2^tsh.  tb == nullptr: the scatter added atomically
+// into the grid fields.
+template <int NF>
+constexpr int part_stride() {
+    return NF <= 1 ? 1 : NF <= 2 ? 2 : NF <= 4 ? 4 : 8;
+}
+
+// The NF values of one partial node (p: the node's first value, aligned to
+// part_stride<NF>() values), in vector accesses; the padding is written as 0.
+template <int NF, typename R>
+__device__ __forceinline__ void load_partial(const R* p, R* v) {
+    constexpr int NS = part_stride<NF>();
+    using V2 = std::conditional_t<sizeof(R) == 4, float2, double2>;
+    if constexpr (sizeof(R) == 4 && NS >= 4) {
+#pragma unroll
+        for (int q = 0; q < NS / 4; ++q) {
+            const float4 t = reinterpret_cast<const float4*>(p)[q];
+            const float e[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (4 * q + j < NF) v[4 * q + j] = R(e[j]);
+        }
+    } else if constexpr (NS >= 2) {
+#pragma unroll
+        for (int q = 0; q < NS / 2; ++q) {
+            const V2 t = reinterpret_cast<const V2*>(p)[q];
+            if (2 * q < NF) v[2 * q] = R(t.x);
+            if (2 * q + 1 < NF) v[2 * q + 1] = R(t.y);
+        }
+    } else {
+        v[0] = p[0];
+    }
+}
+
+template <int NF, typename R>
+__device__ __forceinline__ void store_partial(R* p, const R* v) {
+    constexpr int NS = part_stride<NF>();
+    using V2 = std::conditional_t<sizeof(R) == 4, float2, double2>;
+    auto at = [&](int j) { return j < NF ? v[j] : R(0); };
+    if constexpr (sizeof(R) == 4 && NS >= 4) {
+#pragma unroll
+        for (int q = 0; q < NS / 4; ++q)
+            reinterpret_cast<float4*>(p)[q] =
+                make_float4(float(at(4 * q)), float(at(4 * q + 1)), float(at(4 * q + 2)), float(at(4 * q + 3)));
+    } else if constexpr (NS >= 2) {
+#pragma unroll
+        for (int q = 0; q < NS / 2; ++q) {
+            V2 t;
+            t.x = at(2 * q);
+            t.y = at(2 * q + 1);
+            reinterpret_cast<V2*>(p)[q] = t;
+        }
+    } else {
+        p[0] = v[0];
+    }
+}
 template <typename R>
 struct TilePartials {
     const R* tb;
@@ -162,16 +220,6 @@ struct TilePartials {
     int tsh, ww, wn;
     int tn[3];  // tiles per axis
 };
-
-// Node block of a grid-solve workgroup.  Consecutive workgroups are dispatched
-// to the 8 XCDs in turn; block b is taken as block (b mod 8) per + b / 8 (per =
-// gridDim.x / 8, the grid launched as a multiple of 8 blocks), so each XCD
-// sweeps one contiguous range of nodes (a slab of z layers) and the tile
-// windows that range reads are fetched into that XCD's L2 only.
-__device__ __forceinline__ long long node_block() {
-    const unsigned b = blockIdx.x, per = gridDim.x >> 3;
-    return (long long)((b & 7u) * per + (b >> 3));
-}
 
 // Adds to acc[0, NF) the partials of node n from every tile window that holds
 // it, in a fixed order (tiles with lower indices first).  A window spans
@@ -222,12 +270,16 @@ __device__ __forceinline__ void add_partials(const GridGeom& g, const TilePartia
         koff[c] = (i[0] - (tx << tp.tsh)) +
                   tp.ww * ((i[1] - (ty << tp.tsh)) + (DIM == 3 ? tp.ww * (i[2] - (tz << tp.tsh)) : 0));
     }
+    constexpr int NS = part_stride<NF>();
     R v[NC][NF];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        const R* b = tp.tb + (long long)(slot[c] < 0 ? 0 : slot[c]) * NF * tp.wn + koff[c];
+        if (slot[c] >= 0) {
+            load_partial<NF, R>(tp.tb + ((long long)slot[c] * tp.wn + koff[c]) * NS, v[c]);
+        } else {
 #pragma unroll
-        for (int f = 0; f < NF; ++f) v[c][f] = slot[c] < 0 ? R(0) : b[f * tp.wn];
+            for (int f = 0; f < NF; ++f) v[c][f] = R(0);
+        }
     }
     // lower tile indices first: tz, then ty, then tx ascending
 #pragma unroll
@@ -508,7 +560,7 @@ __global__ __launch_bounds__(256) void k_p2g(GridGeom g, PartPtrs<R> pp, GridPtr
 template <int DIM, typename R>
 __global__ __launch_bounds__(256) void k_grid1(GridGeom g, GridPtrs<R> gp, StepArgs<R> sa,
                                                int ncells, TilePartials<R> tp) {
-    const long long n = node_block() * blockDim.x + threadIdx.x;
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n < ncells && sa.mvl) {
         gp.cell_sm[n] = R(0);
         gp.cell_vol[n] = R(0);
@@ -685,7 +737,7 @@ __global__ __launch_bounds__(256) void k_g2p1_p2g2(GridGeom g, PartPtrs<R> pp, G
 // ---------------------------------------------------------------------------
 template <int DIM, typename R>
 __global__ __launch_bounds__(256) void k_grid2(GridGeom g, GridPtrs<R> gp, R dt, TilePartials<R> tp) {
-    const long long n = node_block() * blockDim.x + threadIdx.x;
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.ni) return;
     const R m = gp.m[n];
     if (m == R(0)) return;  // no mass: p is zero and no G2P weight reads du here

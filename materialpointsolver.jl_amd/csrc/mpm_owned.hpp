@@ -590,7 +590,7 @@ __global__ __launch_bounds__(64) void k_scatter_owned(
         });
         drift_all();
         flush_window<DIM, WW, ZS, WN, NF, R>(g, w0, win, fields,
-                                             tb ? tb + (long long)e * NF * WN : nullptr);
+                                             tb ? tb + (long long)e * part_stride<NF>() * WN : nullptr);
     }
     raise_flag(flags, flag);
 }
@@ -747,7 +747,7 @@ __global__ __launch_bounds__(64) void k_remap_owned(
                                                                                   acc, any);
         drift_all();
         flush_window<DIM, WW, ZS, WN, NF, R>(g, w0, win, fields,
-                                             tb ? tb + (long long)e * NF * WN : nullptr);
+                                             tb ? tb + (long long)e * part_stride<NF>() * WN : nullptr);
     }
 }
 

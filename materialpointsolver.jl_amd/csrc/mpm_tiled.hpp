@@ -158,7 +158,7 @@ __device__ __forceinline__ int paired_offset(int f, int k, int wn) {
 }
 
 // Flush a scatter window.  With a tile-partials slot tb (the tile's place in
-// the tile list), the window is stored there, field-major (tb[f WN + k]), and
+// the tile list), the window is stored there, node-major (store_partial), and
 // the grid solve adds the partials of the up to 2^D windows that hold a node:
 // plain stores and a fixed summation order.  Without (tb == nullptr): global
 // atomic adds of every non-zero node (only window nodes inside the grid can be
@@ -168,9 +168,11 @@ __device__ __forceinline__ void flush_window(const GridGeom& g, const int* w0, l
                                              R* const* fields, R* tb = nullptr) {
     if (tb) {
         for (int k = threadIdx.x; k < WN; k += blockDim.x) {
+            R vals[NF];
 #pragma unroll
             for (int f = 0; f < NF; ++f)
-                tb[f * WN + k] = win[kPairedP2G<DIM, NF, R, WN> ? paired_offset<NF>(f, k, WN) : f * WN + k];
+                vals[f] = win[kPairedP2G<DIM, NF, R, WN> ? paired_offset<NF>(f, k, WN) : f * WN + k];
+            store_partial<NF, R>(tb + (long long)k * part_stride<NF>(), vals);
         }
         return;
     }
@@ -309,6 +311,154 @@ __device__ __forceinline__ void p2g3_window_paired(const Stencil<3, S, float>& s
     });
 }
 
+// Layered scatter windows (FP32 3D uGIMP, the paired layout).  Three windows
+// per wavefront, one per stencil z offset c: layer window c holds what the
+// slots (a, b, c) add, at node (x, y, z - c), so it spans the tile's NB base
+// layers in z (WW x WW x NB nodes) and node0 (the base's offset in a full
+// window) is also the base's offset in every layer window.  A round's lanes
+// hold distinct bases, so slots of one layer can collide across lanes only
+// with slots of the same layer: the three layers are independent chains, and
+// each (a, b) step issues the reads of all three, then the three updates --
+// one LDS round trip per step instead of one per slot (9 per particle instead
+// of 27).  The layers are summed when the window is flushed.
+template <int DIM, int S, int TSH>
+struct WinLayers {
+    using W = Win<DIM, S, TSH>;
+    static constexpr int WNL = W::ZS * W::NB;  // nodes per layer window
+};
+
+// P2G #1 into the layer windows (7 paired fields; layer c at W + c 7 wnl).
+template <int S>
+__device__ __forceinline__ void p2g3_window_layered(const Stencil<3, S, float>& st, float m,
+                                                    float vol, const float* pv, const float* sg,
+                                                    const Gravity<float>& G, lds_ptr<float> W,
+                                                    int node0, int ws, int wnl) {
+    using u64p = volatile __attribute__((address_space(3))) unsigned long long*;
+    auto lo = [](unsigned long long v) { return __uint_as_float(unsigned(v)); };
+    auto hi = [](unsigned long long v) { return __uint_as_float(unsigned(v >> 32)); };
+    auto pack = [](float a, float b) {
+        return (unsigned long long)__float_as_uint(a) |
+               ((unsigned long long)__float_as_uint(b) << 32);
+    };
+    const float mv = -vol;
+    const float sxx = mv * sg[0], sxy = mv * sg[3], sxz = mv * sg[5];
+    const float syy = mv * sg[1], syz = mv * sg[4], szz = mv * sg[2];
+    static_for<S>([&](auto b) {
+        // per c: A = Ny Nz and the force factors P, Q of p2g3_window
+        float A[S], Px[S], Py[S], Pz[S], Qx[S], Qy[S], Qz[S];
+        static_for<S>([&](auto c) {
+            const float a_ = st.N[1][b] * st.N[2][c];
+            const float B = st.dN[1][b] * st.N[2][c];
+            const float C = st.N[1][b] * st.dN[2][c];
+            const float mA = m * a_;
+            A[c] = a_;
+            Px[c] = sxx * a_;
+            Py[c] = sxy * a_;
+            Pz[c] = sxz * a_;
+            Qx[c] = sxy * B + sxz * C + mA * G.g[0];
+            Qy[c] = syy * B + syz * C + mA * G.g[1];
+            Qz[c] = syz * B + szz * C + mA * G.g[2];
+        });
+        static_for<S>([&](auto a) {
+            const float Nx = st.N[0][a], dNx = st.dN[0][a];
+            if (Nx * st.N[1][b] != 0.0f) {
+                const int k = node0 + int(b) * ws + int(a);
+                unsigned long long o0[S], o1[S], o2[S];
+                float o6[S];
+                static_for<S>([&](auto c) {
+                    lds_ptr<float> L = W + int(c) * 7 * wnl;
+                    o0[c] = *reinterpret_cast<u64p>(L + 2 * k);
+                    o1[c] = *reinterpret_cast<u64p>(L + 2 * wnl + 2 * k);
+                    o2[c] = *reinterpret_cast<u64p>(L + 4 * wnl + 2 * k);
+                    o6[c] = L[6 * wnl + k];
+                });
+                static_for<S>([&](auto c) {
+                    lds_ptr<float> L = W + int(c) * 7 * wnl;
+                    const float N = Nx * A[c];
+                    *reinterpret_cast<u64p>(L + 2 * k) = pack(lo(o0[c]) + m * N, hi(o0[c]) + N * pv[0]);
+                    *reinterpret_cast<u64p>(L + 2 * wnl + 2 * k) =
+                        pack(lo(o1[c]) + N * pv[1], hi(o1[c]) + N * pv[2]);
+                    *reinterpret_cast<u64p>(L + 4 * wnl + 2 * k) =
+                        pack(lo(o2[c]) + (dNx * Px[c] + Nx * Qx[c]),
+                             hi(o2[c]) + (dNx * Py[c] + Nx * Qy[c]));
+                    L[6 * wnl + k] = o6[c] + (dNx * Pz[c] + Nx * Qz[c]);
+                });
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        });
+    });
+}
+
+// The momentum scatter of the MUSL re-map into the layer windows (3 fields:
+// (px, py) paired, pz alone).
+template <int S>
+__device__ __forceinline__ void p2g2_window_layered(const Stencil<3, S, float>& st, const float* pv,
+                                                    lds_ptr<float> W, int node0, int ws, int wnl) {
+    using u64p = volatile __attribute__((address_space(3))) unsigned long long*;
+    static_for<S>([&](auto b) {
+        static_for<S>([&](auto a) {
+            const float NxNy = st.N[0][a] * st.N[1][b];
+            if (NxNy != 0.0f) {
+                const int k = node0 + int(b) * ws + int(a);
+                unsigned long long o0[S];
+                float o2[S];
+                static_for<S>([&](auto c) {
+                    lds_ptr<float> L = W + int(c) * 3 * wnl;
+                    o0[c] = *reinterpret_cast<u64p>(L + 2 * k);
+                    o2[c] = L[2 * wnl + k];
+                });
+                static_for<S>([&](auto c) {
+                    lds_ptr<float> L = W + int(c) * 3 * wnl;
+                    const float N = NxNy * st.N[2][c];
+                    const float x = __uint_as_float(unsigned(o0[c])) + N * pv[0];
+                    const float y = __uint_as_float(unsigned(o0[c] >> 32)) + N * pv[1];
+                    *reinterpret_cast<u64p>(L + 2 * k) =
+                        (unsigned long long)__float_as_uint(x) | ((unsigned long long)__float_as_uint(y) << 32);
+                    L[2 * wnl + k] = o2[c] + N * pv[2];
+                });
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        });
+    });
+}
+
+// Flush of the layer windows: window node (x, y, z) is the sum of layer c's
+// node (x, y, z - c) over the layers that hold it (c ascending), then as
+// flush_window (stored as the tile's partials, or added with global atomics).
+template <int DIM, int S, int TSH, int NF, typename R>
+__device__ __forceinline__ void flush_layers(const GridGeom& g, const int* w0, lds_ptr<R> win,
+                                             R* const* fields, R* tb) {
+    using W = Win<DIM, S, TSH>;
+    constexpr int WW = W::WW, ZS = W::ZS, WN = W::WN, NB = W::NB;
+    constexpr int WNL = WinLayers<DIM, S, TSH>::WNL;
+    for (int k = threadIdx.x; k < WN; k += blockDim.x) {
+        int kx, ky, kz;
+        window_node<DIM, WW, ZS>(k, kx, ky, kz);
+        R vals[NF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) vals[f] = R(0);
+#pragma unroll
+        for (int c = 0; c < S; ++c) {
+            const int zl = kz - c;
+            if (zl < 0 || zl >= NB) continue;
+            const int kl = zl * ZS + ky * WW + kx;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) vals[f] += win[c * NF * WNL + paired_offset<NF>(f, kl, WNL)];
+        }
+        if (tb) {
+            store_partial<NF, R>(tb + (long long)k * part_stride<NF>(), vals);
+            continue;
+        }
+        bool any = false;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) any = any || vals[f] != R(0);
+        if (!any) continue;
+        const long long n = node_id(g, w0[0] + kx, w0[1] + ky, w0[2] + kz);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) atomic_add(&fields[f][n], vals[f]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Tile scatters.  MODE 0: P2G #1 (P2G_OS!, utils_OS.jl:424-496): mass,
 // momentum and internal + external force.  MODE 1: the momentum scatter of the
@@ -322,8 +472,8 @@ __device__ __forceinline__ void p2g3_window_paired(const Stencil<3, S, float>& s
 // the same step-start basis; the position is read once and the momentum never
 // goes through memory.
 // ---------------------------------------------------------------------------
-template <int DIM, int S, int TSH, typename R, int MODE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_scatter_tiled(
+template <int DIM, int S, int TSH, typename R, int MODE, bool LAY = false>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LAY ? 3 : 4))) void k_scatter_tiled(
     GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, Gravity<R> G,
     const unsigned* __restrict__ keys, const int* __restrict__ tile_list,
     const int* __restrict__ seg_begin, const int* __restrict__ seg_end,
@@ -334,9 +484,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     constexpr bool REMAP = MODE >= 2;           // gather + move + scatter
     constexpr bool VS = MODE == 3;              // the re-map gathers vs as well
     constexpr int NG = (VS ? 3 : 2) * DIM;      // gathered fields
-    constexpr int NWIN = kScatterWindows;
+    // LAY: three layer windows (WinLayers) and one rank group per round
+    static_assert(!LAY || (DIM == 3 && S == 3 && (MODE == 0 || MODE == 1) &&
+                           kPairedP2G<DIM, NF, R, WN>),
+                  "layer windows: FP32 3D uGIMP scatters");
+    constexpr int NWIN = LAY ? 1 : kScatterWindows;
     static_assert(NWIN == 1 || NWIN == 2, "one or two scatter windows");
-    __shared__ __attribute__((aligned(16))) R win_[NWIN * NF * WN];
+    constexpr int WNL = WinLayers<DIM, S, TSH>::WNL;
+    constexpr int WIN_FLOATS = LAY ? S * NF * WNL : NWIN * NF * WN;
+    __shared__ __attribute__((aligned(16))) R win_[WIN_FLOATS];
     __shared__ unsigned claim_[NWIN * W::NBN];
     __shared__ R gwin[REMAP ? NG * WN : 1];
     R* const* vsf = pic_field(gp, sa);
@@ -370,7 +526,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const int beg = seg_begin[tile], end = seg_end[tile];
         int w0[3];
         tile_window_origin<DIM>(tg, tile, w0);
-        for (int k = threadIdx.x; k < NWIN * NF * WN; k += 64) win[k] = R(0);
+        for (int k = threadIdx.x; k < WIN_FLOATS; k += 64) win[k] = R(0);
         if constexpr (REMAP) {
             __syncthreads();  // the previous tile's gathers from gwin are done
             stage_window<DIM, WW, ZS, WN, NG, R>(g, w0, gfields, gwin);
@@ -519,7 +675,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     }
                     tag += 64u;
                     auto lds_pass = [&] {
-                        if constexpr (MODE == 0 && DIM == 3) {
+                        if constexpr (LAY && MODE == 0) {
+                            p2g3_window_layered<S>(st, m, vol, pv, P.sg, G, win, wb, WW, WNL);
+                        } else if constexpr (LAY) {
+                            p2g2_window_layered<S>(st, pv, win, wb, WW, WNL);
+                        } else if constexpr (MODE == 0 && DIM == 3) {
                             if constexpr (kPairedP2G<DIM, NF, R, WN>)
                                 p2g3_window_paired<S>(st, m, vol, pv, P.sg, G, win + side * NF * WN,
                                                       wb - side * NF * WN, WW, ZS, WN);
@@ -600,11 +760,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             kA = kAn;
             kB = kBn;
         }
+        if constexpr (LAY) {
+            flush_layers<DIM, S, TSH, NF, R>(g, w0, win, fields,
+                                             tb ? tb + (long long)e * part_stride<NF>() * WN : nullptr);
+            continue;
+        }
         if constexpr (NWIN == 2) {
             for (int k = threadIdx.x; k < NF * WN; k += 64) win[k] = win[k] + win[NF * WN + k];
         }
         flush_window<DIM, WW, ZS, WN, NF, R>(g, w0, win, fields,
-                                             tb ? tb + (long long)e * NF * WN : nullptr);
+                                             tb ? tb + (long long)e * part_stride<NF>() * WN : nullptr);
     }
     raise_flag(flags, flag);
 }

@@ -215,7 +215,8 @@ class HIPModel:
 
     def set_option(self, option, value):
         """Runtime option: "sort_interval" (>= 1), "tiles" (bool), "scatter"
-        (the tile scatter form: "owned", "owned_layers" or "rounds"),
+        (the tile scatter form: "rounds", "rounds_layered", "owned" or
+        "owned_layers"),
         "fused_remap" (bool: the one-pass MUSL re-map; off by default),
         "tile_partials" (bool: tile windows stored and summed by the grid solves
         instead of flushed with atomics) or

@@ -202,7 +202,7 @@ def test_slope_20m_learned_return_vs_exact(mpm):
     finally:
         m.close()
     print(f"{len(X)} return samples logged over {nrun} runout steps; branch frequencies "
-          f"(main, edge 12, edge 23, apex) {np.round(Y[:, :4].mean(0), 4)}", flush=True)
+          f"(main, edge 12, edge 23, apex) {np.round(Y[:, :4].mean(0, dtype=np.float64), 4)}", flush=True)
     assert len(X) >= 1_000_000
     group = dict(coords=s0["xi"], velocity_s=s0["vs"], stress=s0["sigma"], strain_s=s0["eps_s"],
                  eqstrain=s0["eps_q"], ekstrain=s0["eps_k"], volume=s0["omega"], mass_s=s0["ms"],

@@ -145,13 +145,25 @@ def test_remap_one_pass_and_split(mpm, orc, fused, scatter):
     _compare(mpm, orc, _c3(mpm, basis="linear", scheme="USL"), 60, 1e-9, sort_interval=5, **kw)
 
 
-@pytest.mark.parametrize("scatter", ["owned", "owned_layers", "rounds"])
+@pytest.mark.parametrize("scatter", ["owned", "owned_layers", "rounds", "rounds_layered"])
 def test_fp32_scatter_forms(mpm, orc, scatter):
     tol = dict(xi=2e-4, vs=5e-4, sigma=2e-3, eps_s=5e-4, eps_q=2e-3, omega=1e-5)
     _compare(mpm, orc, _c3(mpm, eps="FP32", constitutive="druckerprager"), 100, tol,
              sort_interval=10, scatter=scatter)
     _compare(mpm, orc, _c3(mpm, eps="FP32", constitutive="druckerprager"), 100, tol,
              sort_interval=400, scatter=scatter)
+
+
+# The layer-window rank rounds (FP32 3D uGIMP) with tile partials and with the
+# atomic flush, sorted often and rarely (drift out of the windows and onto
+# bases other lanes hold), P2G #1 and the re-map scatter, :a and :v
+@pytest.mark.parametrize("partials", [True, False])
+def test_fp32_rounds_layered(mpm, orc, partials):
+    tol = dict(xi=2e-4, vs=5e-4, sigma=2e-3, eps_s=5e-4, eps_q=2e-3, omega=1e-5)
+    for sort in (5, 400):
+        for va in ("a", "v"):
+            _compare(mpm, orc, _c3(mpm, eps="FP32", constitutive="druckerprager", va=va), 100,
+                     tol, sort_interval=sort, scatter="rounds_layered", tile_partials=partials)
 
 
 # Mohr-Coulomb softening.  2D: cohesion max(c + Hp ϵq, cr) (mohrcoulomb.jl:36;
