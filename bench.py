@@ -99,6 +99,9 @@ def parse():
     ap.add_argument("--scatter", default=None,
                     choices=["rounds", "rounds_layered", "owned", "owned_layers"],
                     help="tile-scatter form (default: the library's)")
+    ap.add_argument("--remap-scatter", default=None,
+                    choices=["rounds", "rounds_layered", "owned", "owned_layers"],
+                    help="the re-map momentum scatter's form (default: the library's)")
     ap.add_argument("--fused-remap", type=int, default=None, choices=[0, 1],
                     help="one-pass MUSL re-map (default: the library's)")
     ap.add_argument("--tile-partials", type=int, default=None, choices=[0, 1],
@@ -125,6 +128,8 @@ def apply_options(a, model):
     """The kernel-variant options given on the command line."""
     if a.scatter is not None:
         model.set_option("scatter", a.scatter)
+    if a.remap_scatter is not None:
+        model.set_option("remap_scatter", a.remap_scatter)
     if a.fused_remap is not None:
         model.set_option("fused_remap", a.fused_remap)
     if a.tile_partials is not None:
@@ -133,7 +138,8 @@ def apply_options(a, model):
 
 def variant_args(a):
     out = []
-    for flag, v in (("--scatter", a.scatter), ("--fused-remap", a.fused_remap),
+    for flag, v in (("--scatter", a.scatter), ("--remap-scatter", a.remap_scatter),
+                    ("--fused-remap", a.fused_remap),
                     ("--tile-partials", a.tile_partials)):
         if v is not None:
             out += [flag, str(v)]
@@ -339,9 +345,16 @@ def main():
     tuned = None
     if not a.no_autotune:
         best, tab, Ti = mpm.autotune(model, sargs, dt, Ti, steps=40, warm=10,
-                                     scatters=mpm.SCATTER_FORMS)
-        tuned = {"best": best, "ms_per_step": {f"sort{k}_{sc if t else 'plain'}": round(s * 1e3, 4)
-                                                for k, t, sc, s in tab}}
+                                     scatters=mpm.SCATTER_FORMS + (("rounds", "rounds"),))
+
+        def _name(sc):
+            return "+".join(sc) if isinstance(sc, tuple) else sc
+
+        tuned = {"best": dict(best, scatter=_name(best["scatter"]) if best["scatter"] else None),
+                 "ms_per_step": {f"sort{k}_{_name(sc) if t else 'plain'}": round(s * 1e3, 4)
+                                 for k, t, sc, s in tab},
+                 "note": "a form name sets P2G #1's scatter (the re-map's automatic); "
+                         "'a+b' sets P2G #1's to a and the re-map's to b"}
         print(f"# autotune: {best}", file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu_baseline:
@@ -389,6 +402,7 @@ def main():
             "active_nodes": n_active,
             "sort_interval": a.sort_interval,
             "kernel_variant": {"scatter": a.scatter or "rounds (default)",
+                               "remap_scatter": a.remap_scatter or "owned (default in FP32 3D)",
                                "fused_remap": bool(a.fused_remap) if a.fused_remap is not None
                                else "off (default)",
                                "tile_partials": bool(a.tile_partials) if a.tile_partials is not None

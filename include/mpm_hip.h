@@ -168,7 +168,8 @@ int mpm_run_graph(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int6
  *                         mpm_profile_kernel_name's order; 0 = every kernel
  *                         (default).  Fewer event pairs per step keep the
  *                         profiled steps closer to unprofiled ones.
- *  MPM_OPT_SCATTER:       form of the tile scatters (P2G and the MUSL re-map):
+ *  MPM_OPT_SCATTER:       form of the tile scatter of P2G #1 (and of the MUSL
+ *                         re-map's, see MPM_OPT_REMAP_SCATTER):
  *                         MPM_SCATTER_ROUNDS (default): rank rounds of
  *                         distinct bases adding into an LDS window;
  *                         MPM_SCATTER_OWNED: each lane owns one stencil base
@@ -190,9 +191,13 @@ int mpm_run_graph(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int6
  *  MPM_OPT_TILE_PARTIALS: 1 (default): the tile scatters store each tile's
  *                         window sums and the grid solves add the windows that
  *                         hold a node, in a fixed order (no flush atomics);
- *                         0: the windows are flushed with global atomics. */
+ *                         0: the windows are flushed with global atomics.
+ *  MPM_OPT_REMAP_SCATTER: form of the MUSL re-map's momentum scatter (an
+ *                         MPM_SCATTER_* value); -1 (default): base-owned
+ *                         (MPM_SCATTER_OWNED) in FP32 3D uGIMP, elsewhere the
+ *                         form of MPM_OPT_SCATTER. */
 enum { MPM_OPT_SORT_INTERVAL = 0, MPM_OPT_TILES = 1, MPM_OPT_PROFILE_MASK = 2, MPM_OPT_SCATTER = 3,
-       MPM_OPT_FUSED_REMAP = 4, MPM_OPT_TILE_PARTIALS = 5 };
+       MPM_OPT_FUSED_REMAP = 4, MPM_OPT_TILE_PARTIALS = 5, MPM_OPT_REMAP_SCATTER = 6 };
 enum { MPM_SCATTER_ROUNDS = 0, MPM_SCATTER_OWNED = 1, MPM_SCATTER_OWNED_LAYERS = 2,
        MPM_SCATTER_ROUNDS_LAYERED = 3 };
 int mpm_set_option(mpm_ctx *ctx, int option, int64_t value);

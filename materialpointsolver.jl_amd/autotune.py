@@ -14,7 +14,8 @@ re-map):
   chains of LDS updates per round (FP32 3D uGIMP; elsewhere ``"rounds"``).
 
 The tile shape (4x4x4 bases in 3D, 8x8 in 2D) and the window size (no slack)
-are compile-time.
+are compile-time.  The MUSL re-map's momentum scatter can take its own form
+(``remap_scatter``; by default the base-owned one in FP32 3D uGIMP).
 
 ``autotune`` times candidate settings on the live model, in place: each
 candidate runs ``warm`` untimed steps and ``steps`` timed steps of the actual
@@ -28,11 +29,24 @@ import time
 SCATTER_FORMS = ("owned", "owned_layers", "rounds", "rounds_layered")
 
 
+def _set_scatter(model, sc):
+    """A candidate scatter form: "form" (P2G #1's form, the re-map's automatic)
+    or (P2G #1's form, the re-map's form)."""
+    if isinstance(sc, tuple):
+        model.set_option("scatter", sc[0])
+        model.set_option("remap_scatter", sc[1])
+    else:
+        model.set_option("scatter", sc)
+        model.set_option("remap_scatter", None)
+
+
 def autotune(model, sargs, dt, Ti, *, sort_intervals=(5, 10, 20, 40), tiles=(True, False),
              scatters=("rounds",), steps=40, warm=10, graph=False, clock=time.perf_counter):
     """Returns (best, table, Ti): best = dict(sort_interval=..., tiles=...,
     scatter=...), table = list of (sort_interval, tiles, scatter, seconds per
-    step), Ti = time after the trials.  The scatter form only matters with
+    step), Ti = time after the trials.  A scatter candidate is a form name
+    (P2G #1's form; the re-map scatter's is automatic) or a pair (P2G #1's
+    form, the re-map scatter's form).  The scatter form only matters with
     tiles; without tiles it is recorded as None."""
     table = []
     for t in tiles:
@@ -40,7 +54,7 @@ def autotune(model, sargs, dt, Ti, *, sort_intervals=(5, 10, 20, 40), tiles=(Tru
             for k in sort_intervals:
                 model.set_option("tiles", t)
                 if sc is not None:
-                    model.set_option("scatter", sc)
+                    _set_scatter(model, sc)
                 model.set_option("sort_interval", k)
                 Ti = model.run(sargs, dt, Ti, warm, graph=graph)
                 model.sync()
@@ -51,6 +65,6 @@ def autotune(model, sargs, dt, Ti, *, sort_intervals=(5, 10, 20, 40), tiles=(Tru
     k, t, sc, _ = min(table, key=lambda r: r[3])
     model.set_option("tiles", t)
     if sc is not None:
-        model.set_option("scatter", sc)
+        _set_scatter(model, sc)
     model.set_option("sort_interval", k)
     return dict(sort_interval=k, tiles=t, scatter=sc), table, Ti

@@ -98,6 +98,7 @@ struct mpm_ctx {
     uint64_t prof_mask = 0;  // kernel ids to time; 0 = all
     bool use_tiles = true;  // LDS-binned scatters when the particles are tile-sorted
     int scatter = MPM_SCATTER_ROUNDS;  // tile scatter form (MPM_OPT_SCATTER)
+    int remap_scatter = -1;            // re-map scatter form, -1: automatic (MPM_OPT_REMAP_SCATTER)
     bool fused_remap = false;         // one-pass MUSL re-map (MPM_OPT_FUSED_REMAP)
     bool tile_partials = true;        // tile scatters store windows (MPM_OPT_TILE_PARTIALS)
     std::vector<ProfRec> recs;
@@ -574,11 +575,19 @@ struct Impl : ImplBase {
         int rc;
         (void)cf;
         const bool tiled = sort.enabled && sort.tiles_valid && c->use_tiles;
-        const bool owned = c->scatter == MPM_SCATTER_OWNED || c->scatter == MPM_SCATTER_OWNED_LAYERS;
         // layer windows for the rank rounds: FP32 3D uGIMP (elsewhere the
         // two-window rounds run)
         constexpr bool kLayerable = DIM == 3 && S == 3 && sizeof(R) == 4;
-        const bool layered = kLayerable && c->scatter == MPM_SCATTER_ROUNDS_LAYERED;
+        // scatter forms of P2G #1 and of the re-map's momentum scatter; the
+        // re-map's follows P2G #1's unless set (MPM_OPT_REMAP_SCATTER), and
+        // is base-owned by default in FP32 3D uGIMP (measured fastest there)
+        const int form1 = c->scatter;
+        const int form2 = c->remap_scatter >= 0 ? c->remap_scatter
+                                                : (kLayerable ? int(MPM_SCATTER_OWNED) : c->scatter);
+        auto is_owned = [](int f) { return f == MPM_SCATTER_OWNED || f == MPM_SCATTER_OWNED_LAYERS; };
+        const bool owned = is_owned(form1), owned2 = is_owned(form2);
+        const bool layered = kLayerable && form1 == MPM_SCATTER_ROUNDS_LAYERED;
+        const bool layered2 = kLayerable && form2 == MPM_SCATTER_ROUNDS_LAYERED;
         constexpr int TSH = DIM == 3 ? 2 : 3;
         // tile partials: the tile scatters store their windows, the grid solves
         // add them up (no flush atomics; MPM_OPT_TILE_PARTIALS)
@@ -602,9 +611,10 @@ struct Impl : ImplBase {
         // register file); FP32 3D P2G #1 sums mass and momentum, then force
         // (MPM_SCATTER_OWNED) or also one z layer per pass
         // (MPM_SCATTER_OWNED_LAYERS); 2D sums everything in one pass.
-        const bool layers = DIM == 3 && (sizeof(R) == 8 || c->scatter == MPM_SCATTER_OWNED_LAYERS);
         auto owned_scatter = [&](auto mode) {
             constexpr int MODE = decltype(mode)::value;
+            const bool layers =
+                DIM == 3 && (sizeof(R) == 8 || (MODE == 0 ? form1 : form2) == MPM_SCATTER_OWNED_LAYERS);
             constexpr int SP_FIELDS = MODE == 0 ? 1 : 0;
             constexpr bool F64 = sizeof(R) == 8;
             if (DIM == 3 && (F64 || layers))
@@ -643,9 +653,9 @@ struct Impl : ImplBase {
             // atomics).  The base-owned one-pass form's FP64 3D uGIMP sums do not
             // fit the register file: there FP64 3D uGIMP keeps the two kernels.
             const bool remap1 = tiled && sa.remap && c->fused_remap &&
-                                (!owned || !(DIM == 3 && S == 3 && sizeof(R) == 8));
+                                (!owned2 || !(DIM == 3 && S == 3 && sizeof(R) == 8));
             if ((rc = c->launch(KID_G2P1, [&] {
-                     if (remap1 && !owned) {
+                     if (remap1 && !owned2) {
                          // the rank-round scatter with the gather and move in its rounds
                          if (sa.va != 0 || sa.pic != R(0))
                              k_scatter_tiled<DIM, S, TSH, R, 3><<<sort.tiled_grid(), 64, 0, s>>>(
@@ -683,9 +693,9 @@ struct Impl : ImplBase {
                 return rc;
             if (tiled && sa.remap && !remap1) {
                 if ((rc = c->launch(KID_P2G2, [&] {
-                         if (owned) {
+                         if (owned2) {
                              owned_scatter(std::integral_constant<int, 1>{});
-                         } else if (layered) {
+                         } else if (layered2) {
                              if constexpr (kLayerable)
                                  k_scatter_tiled<DIM, S, TSH, R, 1, true><<<sort.tiled_grid(), 64, 0, s>>>(
                                      geom, sort.tg, pp, gp, Gv, sort.sorted_keys(), sort.tile_list,
@@ -1315,6 +1325,12 @@ int mpm_set_option(mpm_ctx* c, int option, int64_t value) {
                                       "MPM_SCATTER_OWNED, MPM_SCATTER_OWNED_LAYERS or "
                                       "MPM_SCATTER_ROUNDS_LAYERED");
         c->scatter = int(value);
+    } else if (option == MPM_OPT_REMAP_SCATTER) {
+        if (value != -1 && value != MPM_SCATTER_ROUNDS && value != MPM_SCATTER_OWNED &&
+            value != MPM_SCATTER_OWNED_LAYERS && value != MPM_SCATTER_ROUNDS_LAYERED)
+            return c->fail(MPM_E_ARG, "mpm_set_option: re-map scatter form is -1 (automatic) or "
+                                      "an MPM_SCATTER_* form");
+        c->remap_scatter = int(value);
     } else if (option == MPM_OPT_PROFILE_MASK) {
         if (value < 0 || (kNumKernels < 63 && value >= (int64_t(1) << kNumKernels)))
             return c->fail(MPM_E_ARG, "mpm_set_option: profile mask names no such kernel");

@@ -511,10 +511,9 @@ __global__ __launch_bounds__(64) void k_scatter_owned(
     scatter_fields<DIM, R, MODE, NF>(gp, fields);
     const int lane = threadIdx.x;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    // the lane's base inside the tile (x fastest), and its node in the window
-    const int lx = lane & (NB - 1);
-    const int ly = (lane >> TSH) & (NB - 1);
-    const int lz = DIM == 3 ? lane >> (2 * TSH) : 0;
+    // the lane's base inside the tile (tile_local order), and its node in the window
+    int lx, ly, lz;
+    tile_local_xyz<DIM>(TSH, lane, lx, ly, lz);
     const int wbase0 = DIM == 3 ? lz * ZS + ly * WW + lx : ly * WW + lx;
     unsigned tag = unsigned(lane);
     int flag = 0;
@@ -640,9 +639,8 @@ __global__ __launch_bounds__(64) void k_remap_owned(
     const Gravity<R> G0{};  // unused by the momentum scatter
     const int lane = threadIdx.x;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    const int lx = lane & (NB - 1);
-    const int ly = (lane >> TSH) & (NB - 1);
-    const int lz = DIM == 3 ? lane >> (2 * TSH) : 0;
+    int lx, ly, lz;
+    tile_local_xyz<DIM>(TSH, lane, lx, ly, lz);
     const int wbase0 = DIM == 3 ? lz * ZS + ly * WW + lx : ly * WW + lx;
     unsigned tag = unsigned(lane);
     int e0, e1, est;

@@ -66,8 +66,7 @@ __global__ __launch_bounds__(256) void k_sort_keys(GridGeom g, SortKeyGeom kg,
     const int tm = (1 << kg.tsh) - 1;
     const unsigned tile = unsigned(b[0] >> kg.tsh) +
                           unsigned(kg.tx) * (unsigned(b[1] >> kg.tsh) + unsigned(kg.ty) * unsigned(b[2] >> kg.tsh));
-    const unsigned local = unsigned(b[0] & tm) +
-                           unsigned(tm + 1) * (unsigned(b[1] & tm) + unsigned(tm + 1) * unsigned(b[2] & tm));
+    const unsigned local = unsigned(tile_local<DIM>(kg.tsh, b[0] & tm, b[1] & tm, b[2] & tm));
     keys[i] = (tile << kTileShift) | (min(rank, (1u << kRankBits) - 1) << kLocalBits) | local;
     rank_out[i] = int(rank);
 }
@@ -91,7 +90,8 @@ __global__ __launch_bounds__(64) void k_tile_layout(SortKeyGeom kg, const unsign
     if (t >= ntiles) return;
     const int tm = (1 << kg.tsh) - 1;
     const int cx = t % kg.tx, cy = (t / kg.tx) % kg.ty, cz = DIM == 3 ? t / (kg.tx * kg.ty) : 0;
-    const int lx = lane & tm, ly = (lane >> kg.tsh) & tm, lz = DIM == 3 ? lane >> (2 * kg.tsh) : 0;
+    int lx, ly, lz;
+    tile_local_xyz<DIM>(kg.tsh, lane, lx, ly, lz);
     const int b0 = (cx << kg.tsh) + lx, b1 = (cy << kg.tsh) + ly, b2 = DIM == 3 ? (cz << kg.tsh) + lz : 0;
     const bool valid = b0 < kg.nb[0] && b1 < kg.nb[1] && b2 < kg.nb[2];
     const int c = valid ? int(count[b0 + kg.nb[0] * (b1 + kg.nb[1] * b2)]) : 0;

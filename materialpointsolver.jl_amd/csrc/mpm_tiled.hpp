@@ -40,6 +40,37 @@ namespace mpm {
 constexpr int kRankBits = 5;   // ranks 0..31 (a fuller dual cell shares rank 31)
 constexpr int kLocalBits = 6;  // 64 bases per tile
 constexpr int kTileShift = kRankBits + kLocalBits;
+
+// Local index of a stencil base inside its tile (bx, by, bz in [0, 2^tsh)):
+// the low bits of the sort key, so the lanes of a round hold the group's bases
+// in this order.  3D tiles of 4x4x4 bases: l = bz + 4 (bx & 1) + 8 (by & 1) +
+// 16 (bx >> 1) + 32 (by >> 1), so every 16 consecutive lanes of a full round
+// hold a 2x2x4 block of bases.  In a 6x6x6 window (node x + 6 y + 36 z) one
+// stencil slot of such a block lands on x + 6 y + 4 z mod 16 = 16 distinct
+// values (x, y in {0, 1}, z in 0..3, plus the slot's constant offset), i.e.
+// on 16 distinct bank pairs of a 64-bit LDS access of 16 lanes; with x
+// fastest, two of a 4x4 block's 16 slots share a bank pair.  Otherwise (2D,
+// other tile sizes) x fastest.
+template <int DIM>
+__device__ __forceinline__ int tile_local(int tsh, int bx, int by, int bz) {
+    if (DIM == 3 && tsh == 2)
+        return bz + 4 * (bx & 1) + 8 * (by & 1) + 16 * (bx >> 1) + 32 * (by >> 1);
+    return bx + ((by + (DIM == 3 ? bz << tsh : 0)) << tsh);
+}
+
+template <int DIM>
+__device__ __forceinline__ void tile_local_xyz(int tsh, int l, int& bx, int& by, int& bz) {
+    if (DIM == 3 && tsh == 2) {
+        bz = l & 3;
+        bx = ((l >> 2) & 1) | (((l >> 4) & 1) << 1);
+        by = ((l >> 3) & 1) | (((l >> 5) & 1) << 1);
+        return;
+    }
+    const int tm = (1 << tsh) - 1;
+    bx = l & tm;
+    by = (l >> tsh) & tm;
+    bz = DIM == 3 ? l >> (2 * tsh) : 0;
+}
 // Bases of slack on each side of a window.  0 measured fastest on the 3D
 // column: with one base of slack the 7-field scatter window grows from 6^3 to
 // 8^3 nodes, fewer wavefronts fit per CU, and P2G #1 and the re-map gather

@@ -215,8 +215,9 @@ class HIPModel:
 
     def set_option(self, option, value):
         """Runtime option: "sort_interval" (>= 1), "tiles" (bool), "scatter"
-        (the tile scatter form: "rounds", "rounds_layered", "owned" or
-        "owned_layers"),
+        (P2G #1's tile scatter form: "rounds", "rounds_layered", "owned" or
+        "owned_layers"), "remap_scatter" (the re-map momentum scatter's form,
+        None = automatic: "owned" in FP32 3D uGIMP, else as "scatter"),
         "fused_remap" (bool: the one-pass MUSL re-map; off by default),
         "tile_partials" (bool: tile windows stored and summed by the grid solves
         instead of flushed with atomics) or
@@ -232,10 +233,15 @@ class HIPModel:
                 mask |= 1 << names.index(name)
             self._check(self.lib.mpm_set_option(self.ctx, capi.MPM_OPT_PROFILE_MASK, mask))
             return
-        if option == "scatter":
-            if value not in capi.SCATTER:
-                raise ValueError(f"scatter form {value!r}; forms: {sorted(capi.SCATTER)}")
-            self._check(self.lib.mpm_set_option(self.ctx, capi.MPM_OPT_SCATTER, capi.SCATTER[value]))
+        if option in ("scatter", "remap_scatter"):
+            if option == "remap_scatter" and value is None:
+                code = -1  # automatic
+            elif value in capi.SCATTER:
+                code = capi.SCATTER[value]
+            else:
+                raise ValueError(f"{option} form {value!r}; forms: {sorted(capi.SCATTER)}")
+            opt = capi.MPM_OPT_SCATTER if option == "scatter" else capi.MPM_OPT_REMAP_SCATTER
+            self._check(self.lib.mpm_set_option(self.ctx, opt, code))
             return
         code = {"sort_interval": capi.MPM_OPT_SORT_INTERVAL, "tiles": capi.MPM_OPT_TILES,
                 "fused_remap": capi.MPM_OPT_FUSED_REMAP,

@@ -18,16 +18,18 @@ def _rel(a, b):
 
 def _compare(mpm, orc, scene, nsteps, tol, sort_interval=0, keys=("xi", "vs", "sigma", "eps_s",
                                                                    "eps_q", "omega"),
-             scatter=None, fused_remap=None, tile_partials=None):
+             scatter=None, fused_remap=None, tile_partials=None, remap_scatter=None):
     args, grid, mp, attr, bc = scene
     print(f"case: dim={grid.dim} eps={mp.ϵ} np={mp.np} const={args.constitutive} va={args.va} "
           f"MVL={args.MVL} NIC={grid.NIC} steps={nsteps} sort={sort_interval} "
           f"scatter={scatter or 'default'} fused_remap={fused_remap} "
-          f"tile_partials={tile_partials}", flush=True)
+          f"tile_partials={tile_partials} remap_scatter={remap_scatter or 'default'}", flush=True)
     model = mpm.HIPModel(grid, mp, attr, bc, sort_interval=sort_interval)
     try:
         if scatter is not None:
             model.set_option("scatter", scatter)
+        if remap_scatter is not None:
+            model.set_option("remap_scatter", remap_scatter)
         if fused_remap is not None:
             model.set_option("fused_remap", fused_remap)
         if tile_partials is not None:
@@ -131,7 +133,7 @@ def test_remap_one_pass_and_split(mpm, orc, fused, scatter):
     # momentum-scatter pair, both against the oracle; FP64 in 2D and 3D, FP32
     # in 3D; sort interval 400 adds drifted particles
     tol = dict(xi=2e-4, vs=5e-4, sigma=2e-3, eps_s=5e-4, eps_q=2e-3, omega=1e-5)
-    kw = dict(fused_remap=fused, scatter=scatter)
+    kw = dict(fused_remap=fused, scatter=scatter, remap_scatter=scatter)
     for sort in (5, 400):
         _compare(mpm, orc, _c2(mpm, constitutive="druckerprager", cohesion=200.0), 200, 1e-9,
                  sort_interval=sort, **kw)
@@ -152,6 +154,18 @@ def test_fp32_scatter_forms(mpm, orc, scatter):
              sort_interval=10, scatter=scatter)
     _compare(mpm, orc, _c3(mpm, eps="FP32", constitutive="druckerprager"), 100, tol,
              sort_interval=400, scatter=scatter)
+
+
+# The re-map's momentum scatter in each form, with P2G #1 in the rank rounds
+# (FP32 3D uGIMP takes the base-owned form by default), sorted often and rarely
+@pytest.mark.parametrize("remap_scatter", ["rounds", "owned", "owned_layers", "rounds_layered"])
+def test_fp32_remap_scatter_forms(mpm, orc, remap_scatter):
+    tol = dict(xi=2e-4, vs=5e-4, sigma=2e-3, eps_s=5e-4, eps_q=2e-3, omega=1e-5)
+    for sort in (10, 400):
+        _compare(mpm, orc, _c3(mpm, eps="FP32", constitutive="druckerprager"), 100, tol,
+                 sort_interval=sort, scatter="rounds", remap_scatter=remap_scatter)
+    _compare(mpm, orc, _c3(mpm), 60, 1e-9, sort_interval=5, scatter="rounds",
+             remap_scatter=remap_scatter)
 
 
 # The layer-window rank rounds (FP32 3D uGIMP) with tile partials and with the

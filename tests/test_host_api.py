@@ -142,7 +142,8 @@ def test_autotune_picks_the_fastest_and_leaves_it_set(mpm):
     m = FakeModel()
     best, table, Ti = mpm.autotune(m, None, 0.5, 0.0, steps=4, warm=1, clock=lambda: m.now)
     assert best == dict(sort_interval=10, tiles=True, scatter="rounds")
-    assert m.opts == {"sort_interval": 10, "tiles": True, "scatter": "rounds"}
+    assert m.opts == {"sort_interval": 10, "tiles": True, "scatter": "rounds",
+                      "remap_scatter": None}
     assert len(table) == 8 and abs(min(t[3] for t in table) - 1.3) < 1e-12
     assert Ti == 8 * 5 * 0.5  # every trial advanced the clock
     # with every scatter form: 4 forms x 4 intervals with tiles + 4 without
@@ -150,8 +151,16 @@ def test_autotune_picks_the_fastest_and_leaves_it_set(mpm):
     best, table, Ti = mpm.autotune(m, None, 0.5, 0.0, steps=4, warm=1, clock=lambda: m.now,
                                    scatters=mpm.SCATTER_FORMS)
     assert best == dict(sort_interval=10, tiles=True, scatter="owned_layers")
-    assert m.opts == {"sort_interval": 10, "tiles": True, "scatter": "owned_layers"}
+    assert m.opts == {"sort_interval": 10, "tiles": True, "scatter": "owned_layers",
+                      "remap_scatter": None}
     assert len(table) == 20 and abs(min(t[3] for t in table) - 0.9) < 1e-12
+    # a (P2G #1, re-map scatter) pair sets both forms
+    m = FakeModel()
+    best, table, Ti = mpm.autotune(m, None, 0.5, 0.0, steps=4, warm=1, clock=lambda: m.now,
+                                   scatters=(("owned_layers", "rounds"), "rounds"),
+                                   sort_intervals=(10,))
+    assert best["scatter"] == ("owned_layers", "rounds")
+    assert m.opts["scatter"] == "owned_layers" and m.opts["remap_scatter"] == "rounds"
 
 
 def test_snapshot_layout_and_cadence(mpm, monkeypatch, tmp_path):
