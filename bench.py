@@ -116,7 +116,7 @@ TRIAD_BYTES = 256 << 20
 KERNEL_RE = {
     "k_p2g": r"mpm::k_p2g<|mpm::k_scatter_(tiled|owned)<\d, \d, \d, \w+, 0[,>]",
     "k_grid1": r"mpm::k_grid1<",
-    "k_g2p1_p2g2": (r"mpm::k_g2p1(_p2g2)?(_tiled)?<|mpm::k_scatter_tiled<\d, \d, \d, \w+, [23]>"
+    "k_g2p1_p2g2": (r"mpm::k_g2p1(_p2g2)?(_tiled)?<|mpm::k_scatter_tiled<\d, \d, \d, \w+, [23][,>]"
                     r"|mpm::k_remap_owned<"),
     "k_p2g2": r"mpm::k_scatter_(tiled|owned)<\d, \d, \d, \w+, 1[,>]",
     "k_grid2": r"mpm::k_grid2<",

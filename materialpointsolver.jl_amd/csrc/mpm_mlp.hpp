@@ -272,6 +272,48 @@ __global__ __launch_bounds__(64) void k_gemm(int M, int N, int K, const __bf16* 
     }
 }
 
+// Split-K form of k_gemm for the weight gradients (M, N <= 128, K = the batch):
+// slice z of gridDim.z sums k in [z kc, (z + 1) kc) into ws[z][i][j]; then
+// k_splitk_reduce adds the slices in slice order into C (deterministic).
+template <bool TA, bool TB>
+__global__ __launch_bounds__(64) void k_gemm_splitk(int M, int N, int K, int kc,
+                                                    const __bf16* __restrict__ A, int lda,
+                                                    const __bf16* __restrict__ B, int ldb,
+                                                    float* __restrict__ ws) {
+    const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32, l = threadIdx.x, hi = l >> 5;
+    const int ia = i0 + (l & 31), jb = j0 + (l & 31);
+    const int kb = blockIdx.z * kc, ke = min(K, kb + kc);
+    f32x16 acc = {};
+    for (int k0 = kb; k0 < ke; k0 += 16) {
+        bf16x8 a, b;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            const int k = k0 + 8 * hi + v;
+            a[v] = (ia < M && k < ke) ? (TA ? A[(long long)k * lda + ia] : A[(long long)ia * lda + k])
+                                      : (__bf16)0.0f;
+            b[v] = (jb < N && k < ke) ? (TB ? B[(long long)jb * ldb + k] : B[(long long)k * ldb + jb])
+                                      : (__bf16)0.0f;
+        }
+        acc = mfma(a, b, acc);
+    }
+    float* w = ws + (long long)blockIdx.z * M * N;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int i = i0 + 8 * (r >> 2) + 4 * hi + (r & 3), j = jb;
+        if (i < M && j < N) w[(long long)i * N + j] = acc[r];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_splitk_reduce(const float* __restrict__ ws, int slices, int M,
+                                                       int N, float* __restrict__ c, int ldc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M * N) return;
+    const int i = t / N, j = t - i * N;
+    float s = 0.0f;
+    for (int z = 0; z < slices; ++z) s += ws[(long long)z * M * N + t];
+    c[(long long)i * ldc + j] = s;
+}
+
 // Output delta of the mean squared error over normalised targets:
 // d = 2 (o - y) / (B * 7); loss partial sums per block.
 __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ o, const float* __restrict__ y,
@@ -297,22 +339,50 @@ __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ o, const 
 }
 
 // Column sums of a rows x cols FP32 matrix (bias gradients), added to g.
-__global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ m, int rows, int cols,
-                                                float* __restrict__ g) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cols) return;
+// Column sums of a row-major rows x cols matrix (bias gradients), in two
+// deterministic stages.  Stage 1: block b sums rows [b RB, (b + 1) RB) of every
+// column, its threads laid out as row groups x columns (consecutive threads on
+// consecutive columns, so each row group reads whole rows), eight independent
+// loads in flight per thread, the groups combined in LDS in a fixed order;
+// part[b][j] holds the block's sum.  Stage 2: one thread per column adds the
+// blocks' sums in block order.  cols <= 256.
+constexpr int kColsumBlocks = 64;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_colsum_part(const T* __restrict__ m, int rows, int cols,
+                                                     float* __restrict__ part) {
+    __shared__ float red[256];
+    const int groups = 256 / cols;
+    const int t = threadIdx.x, grp = t / cols, j = t - grp * cols;
+    const int RB = (rows + gridDim.x - 1) / gridDim.x;
+    const int r0 = blockIdx.x * RB, r1 = min(rows, r0 + RB);
     float s = 0.0f;
-    for (int i = 0; i < rows; ++i) s += m[(long long)i * cols + j];
-    g[j] = s;
+    if (grp < groups) {
+        int r = r0 + grp;
+        for (; r + 7 * groups < r1; r += 8 * groups) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = float(m[(long long)(r + u * groups) * cols + j]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += v[u];
+        }
+        for (; r < r1; r += groups) s += float(m[(long long)r * cols + j]);
+    }
+    red[t] = s;
+    __syncthreads();
+    if (t < cols) {
+        float acc = 0.0f;
+        for (int g2 = 0; g2 < groups; ++g2) acc += red[g2 * cols + t];
+        part[blockIdx.x * cols + t] = acc;
+    }
 }
 
-// Column sums of a BF16 delta (hidden-layer bias gradients).
-__global__ __launch_bounds__(256) void k_colsum_bf16(const __bf16* __restrict__ m, int rows,
-                                                     int cols, float* __restrict__ g) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_colsum_final(const float* __restrict__ part, int nblocks,
+                                                      int cols, float* __restrict__ g) {
+    const int j = threadIdx.x;
     if (j >= cols) return;
     float s = 0.0f;
-    for (int i = 0; i < rows; ++i) s += float(m[(long long)i * cols + j]);
+    for (int b = 0; b < nblocks; ++b) s += part[b * cols + j];
     g[j] = s;
 }
 

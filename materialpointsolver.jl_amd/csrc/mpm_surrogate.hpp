@@ -130,6 +130,9 @@ struct mpm_mlp {
     __bf16 *d1 = nullptr, *d2 = nullptr, *d3 = nullptr, *d4 = nullptr;
     float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr, *o = nullptr, *yb = nullptr;
     float *dsum = nullptr, *loss = nullptr;
+    float* csum = nullptr;       // column-sum partials (kColsumBlocks x NH)
+    float* splitk_ws = nullptr;  // split-K slices of a weight gradient (kSplitK x NH x NH)
+    static constexpr int kSplitK = 32;
     float *X = nullptr, *Y = nullptr;  // dataset rows on the device
     long long ndata = 0, data_cap = 0;
     long long adam_t = 0;
@@ -178,6 +181,8 @@ struct mpm_mlp {
         if (e == hipSuccess) e = alloc(&yb, (size_t)cap * NOUT);
         if (e == hipSuccess) e = alloc(&dsum, (size_t)cap * NOUT);
         if (e == hipSuccess) e = alloc(&loss, 1);
+        if (e == hipSuccess) e = alloc(&csum, (size_t)mpm::mlp::kColsumBlocks * NH);
+        if (e == hipSuccess) e = alloc(&splitk_ws, (size_t)kSplitK * NH * NH);
         if (e == hipSuccess) e = hipMemset(P, 0, sizeof(float) * NTOTAL);
         if (e == hipSuccess) e = hipMemset(m, 0, sizeof(float) * NPARAM);
         if (e == hipSuccess) e = hipMemset(v, 0, sizeof(float) * NPARAM);
@@ -215,8 +220,22 @@ struct mpm_mlp {
     template <bool TA, bool TB, int EPI>
     void gemm(int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb,
               mpm::mlp::GemmEpi ep) {
+        using namespace mpm::mlp;
         dim3 grid((N + 31) / 32, (M + 31) / 32);
-        mpm::mlp::k_gemm<TA, TB, EPI><<<grid, 64, 0, stream>>>(M, N, K, A, lda, B, ldb, ep);
+        // the weight gradients (few output tiles, K = the batch): split K over
+        // kSplitK slices so that the whole chip works on them
+        if constexpr (EPI == EPI_STORE) {
+            if (K >= 1024 && (long long)M * N <= (long long)NH * NH) {
+                const int kc = ((K + kSplitK - 1) / kSplitK + 15) / 16 * 16;
+                const int slices = (K + kc - 1) / kc;
+                k_gemm_splitk<TA, TB><<<dim3(grid.x, grid.y, slices), 64, 0, stream>>>(
+                    M, N, K, kc, A, lda, B, ldb, splitk_ws);
+                k_splitk_reduce<<<(M * N + 255) / 256, 256, 0, stream>>>(splitk_ws, slices, M, N, ep.c,
+                                                                          ep.ldc);
+                return;
+            }
+        }
+        k_gemm<TA, TB, EPI><<<grid, 64, 0, stream>>>(M, N, K, A, lda, B, ldb, ep);
     }
 
     // forward + backward on the B rows in xb / yb; gradient into G, loss added to *lossp
@@ -237,26 +256,34 @@ struct mpm_mlp {
         // layer 4
         ep = GemmEpi{nullptr, nullptr, nullptr, G + OW4, NH};
         gemm<true, false, EPI_STORE>(NOUT, NH, B, d4, NOUT, h3, NH, ep);
-        k_colsum<<<1, 256, 0, stream>>>(dsum, B, NOUT, G + OB4);
+        colsum<float>(dsum, B, NOUT, G + OB4);
         ep = GemmEpi{nullptr, z3, d3, nullptr, 0};
         gemm<false, false, EPI_DELTA>(B, NH, NOUT, d4, NOUT, W4, NH, ep);
         // layer 3
         ep = GemmEpi{nullptr, nullptr, nullptr, G + OW3, NH};
         gemm<true, false, EPI_STORE>(NH, NH, B, d3, NH, h2, NH, ep);
-        k_colsum_bf16<<<1, 256, 0, stream>>>(d3, B, NH, G + OB3);
+        colsum<__bf16>(d3, B, NH, G + OB3);
         ep = GemmEpi{nullptr, z2, d2, nullptr, 0};
         gemm<false, false, EPI_DELTA>(B, NH, NH, d3, NH, W3, NH, ep);
         // layer 2
         ep = GemmEpi{nullptr, nullptr, nullptr, G + OW2, NH};
         gemm<true, false, EPI_STORE>(NH, NH, B, d2, NH, h1, NH, ep);
-        k_colsum_bf16<<<1, 256, 0, stream>>>(d2, B, NH, G + OB2);
+        colsum<__bf16>(d2, B, NH, G + OB2);
         ep = GemmEpi{nullptr, z1, d1, nullptr, 0};
         gemm<false, false, EPI_DELTA>(B, NH, NH, d2, NH, W2, NH, ep);
         // layer 1
         ep = GemmEpi{nullptr, nullptr, nullptr, G + OW1, NIN};
         gemm<true, false, EPI_STORE>(NH, NIN, B, d1, NH, xb, NIN, ep);
-        k_colsum_bf16<<<1, 256, 0, stream>>>(d1, B, NH, G + OB1);
+        colsum<__bf16>(d1, B, NH, G + OB1);
         return hipGetLastError();
+    }
+
+    template <typename T>
+    void colsum(const T* m, int rows, int cols, float* g) {
+        using namespace mpm::mlp;
+        const int nb = rows < 256 ? 1 : kColsumBlocks;
+        k_colsum_part<T><<<nb, 256, 0, stream>>>(m, rows, cols, csum);
+        k_colsum_final<<<1, 256, 0, stream>>>(csum, nb, cols, g);
     }
 
     void batch(int B, unsigned long long seed, long long step, bool sequential) {

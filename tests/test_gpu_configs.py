@@ -145,11 +145,21 @@ def test_slope_20m_k0_and_runout(mpm):
     x0 = np.asarray(mp.ξ, dtype=np.float64)
     m0 = float(np.sum(np.asarray(mp.Ω, dtype=np.float64) * np.asarray(mp.ρs, dtype=np.float64)))
     dt = lay["dt"]
+    ra = lay["runout_args"]
+    nrun = int(round((ra.Ttol - ra.Te) / dt))  # the 30 s runout
     m = mpm.HIPModel(grid, mp, attr, bc, sort_interval=5)
     try:
         Ti = m.run(m.step_args(args), dt, 0.0, lay["k0_steps"], graph=True)
         k0 = m.download()
-        Ti = m.run(m.step_args(lay["runout_args"]), dt, Ti, 2_000, graph=True)
+        sa = m.step_args(ra)
+        done = 0
+        while done < nrun:
+            k = min(2_000, nrun - done)
+            Ti = m.run(sa, dt, Ti, k, graph=True)
+            done += k
+            if done == 2_000:
+                early = m.download()
+            print(f"runout: {done} of {nrun} steps, t = {Ti:.2f} s", flush=True)
         out = m.download()
     finally:
         m.close()
@@ -173,10 +183,21 @@ def test_slope_20m_k0_and_runout(mpm):
           f"sigma_xz / (rho g d sin) median {np.median(sxz):.3f}", flush=True)
     assert 0.85 < np.median(szz) < 1.15
     assert 0.7 < np.median(sxz) < 1.3
+    dx2 = early["xi"][:, 0] - x0[:, 0]
     dx = out["xi"][:, 0] - x0[:, 0]
-    print(f"runout after 2000 steps ({2000 * dt:.2f} s): mean dx {dx.mean():.3f} m, max {dx.max():.2f} m",
+    print(f"runout after 2000 steps ({2000 * dt:.2f} s): mean dx {dx2.mean():.3f} m, max {dx2.max():.2f} m",
           flush=True)
-    assert dx.mean() > 0.0  # the failing mass moves downslope
+    f0 = np.quantile(x0[:, 0], 0.999)
+    front = np.quantile(out["xi"][:, 0], 0.999) - f0
+    print(f"runout after {nrun} steps ({Ti - ra.Te:.2f} s): mean dx {dx.mean():.3f} m, max {dx.max():.2f} m, "
+          f"front advance (99.9 % quantile) {front:.2f} m", flush=True)
+    assert dx2.mean() > 0.0 and dx.mean() > dx2.mean()  # the failing mass keeps moving downslope
+    # no particle sinks into the rigid bed (more than one cell below its surface)
+    xe = out["xi"]
+    zb_end = mpm.scenes._bilinear(lay["bed"], -2 * h, h, -2 * h, h, xe[:, 0], xe[:, 1])
+    sink = (zb_end - xe[:, 2]).max()
+    print(f"deepest particle below the bed surface: {sink:.3f} m", flush=True)
+    assert sink < h
 
 
 def test_slope_20m_learned_return_vs_exact(mpm):
