@@ -339,6 +339,9 @@ def main():
         print(f"#   {name:14s} {k['avg_us']:9.1f} us x {k['launches']:5d}  {g}",
               file=sys.stderr, flush=True)
 
+    # the achievable streaming bandwidth on this GPU (triad a = b + c, 1 GiB arrays)
+    probe_gbs = mpm.Tpeak(nbytes=1 << 30, reps=5)
+    print(f"# triad probe: {probe_gbs:.0f} GB/s", file=sys.stderr, flush=True)
     # k_g2p1_p2g2's entry already holds the re-map scatter's 12 B per node
     step_bytes = sum(ALGO_BYTES[k][0] * n + ALGO_BYTES[k][1] * n_active for k in ALGO_BYTES
                      if k != "k_p2g2")
@@ -409,6 +412,7 @@ def main():
                                else "on (default)"},
             "step_algorithmic_bytes": step_bytes,
             "step_hbm_fraction": step_bytes / (elapsed / a.steps) / (HBM_PEAK_GBS * 1e9),
+            "hbm_triad_probe_gbs": probe_gbs,
             "kernels_us": {k: round(v["avg_us"], 2) for k, v in kernels.items()},
             "kernels_us_window": f"the {a.steps} steps after the timed region, events on every "
                                  f"kernel ({ms_all_events:.4f} ms/step)",

@@ -23,7 +23,6 @@
 // travels with the particles (pid) so downloads return the caller's order.
 #pragma once
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <string>
 #include <type_traits>
@@ -127,39 +126,121 @@ __global__ __launch_bounds__(64) void k_tile_layout(SortKeyGeom kg, const unsign
 }
 
 // Tile segments of the sorted order from the scanned tile sizes.
+// Scan of the tile sizes (hand-written, three launches): for every tile t,
+// starts[t] = the particles of the tiles below t (exclusive sum of totals) and
+// slot_of[t] = the non-empty tiles below t (the tile's place in the compacted
+// tile list and its tile-partials slot), -1 for an empty tile; *tile_count =
+// the non-empty tiles.  Stage 1 sums each block of kScanBlock tiles, stage 2
+// scans the block sums in one workgroup (chunk by chunk, with a carry), stage
+// 3 scans each block again and adds its offset.  Integer sums: the result does
+// not depend on the order.
+constexpr int kScanBlock = 1024;
+
+// Inclusive scan of (a, b) over a workgroup of kScanBlock threads; returns the
+// workgroup's totals in ta, tb.  red holds 2 (kScanBlock / 64) ints.
+__device__ __forceinline__ void block_scan2(int& a, int& b, int* red, int& ta, int& tb) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int NW = kScanBlock / 64;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int ua = __shfl_up(a, d), ub = __shfl_up(b, d);
+        if (lane >= d) {
+            a += ua;
+            b += ub;
+        }
+    }
+    if (lane == 63) {
+        red[wave] = a;
+        red[NW + wave] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        int wa = threadIdx.x < NW ? red[threadIdx.x] : 0;
+        int wb = threadIdx.x < NW ? red[NW + threadIdx.x] : 0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int ua = __shfl_up(wa, d), ub = __shfl_up(wb, d);
+            if (threadIdx.x >= d) {
+                wa += ua;
+                wb += ub;
+            }
+        }
+        if (threadIdx.x < NW) {
+            red[threadIdx.x] = wa;  // inclusive over waves
+            red[NW + threadIdx.x] = wb;
+        }
+    }
+    __syncthreads();
+    if (wave > 0) {
+        a += red[wave - 1];
+        b += red[NW + wave - 1];
+    }
+    ta = red[NW - 1];
+    tb = red[2 * NW - 1];
+    __syncthreads();  // red is reused by the caller's next scan
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_tile_scan_sums(const int* __restrict__ totals, int ntiles,
+                                                               int* __restrict__ bsum) {
+    __shared__ int red[2 * (kScanBlock / 64)];
+    const int t = blockIdx.x * kScanBlock + threadIdx.x;
+    int a = t < ntiles ? totals[t] : 0;
+    int b = a > 0 ? 1 : 0;
+    int ta, tb;
+    block_scan2(a, b, red, ta, tb);
+    if (threadIdx.x == 0) {
+        bsum[2 * blockIdx.x] = ta;
+        bsum[2 * blockIdx.x + 1] = tb;
+    }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_tile_scan_blocks(int* __restrict__ bsum, int nblocks,
+                                                                 int* __restrict__ tile_count) {
+    __shared__ int red[2 * (kScanBlock / 64)];
+    int ca = 0, cb = 0;  // carry: sums of the earlier chunks
+    for (int c0 = 0; c0 < nblocks; c0 += kScanBlock) {
+        const int k = c0 + int(threadIdx.x);
+        const int xa = k < nblocks ? bsum[2 * k] : 0, xb = k < nblocks ? bsum[2 * k + 1] : 0;
+        int a = xa, b = xb, ta, tb;
+        block_scan2(a, b, red, ta, tb);
+        if (k < nblocks) {
+            bsum[2 * k] = ca + a - xa;  // exclusive
+            bsum[2 * k + 1] = cb + b - xb;
+        }
+        ca += ta;
+        cb += tb;
+    }
+    if (threadIdx.x == 0) *tile_count = cb;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_tile_scan_apply(const int* __restrict__ totals, int ntiles,
+                                                                const int* __restrict__ bsum,
+                                                                int* __restrict__ starts,
+                                                                int* __restrict__ slot_of) {
+    __shared__ int red[2 * (kScanBlock / 64)];
+    const int t = blockIdx.x * kScanBlock + threadIdx.x;
+    const int xa = t < ntiles ? totals[t] : 0, xb = xa > 0 ? 1 : 0;
+    int a = xa, b = xb, ta, tb;
+    block_scan2(a, b, red, ta, tb);
+    if (t < ntiles) {
+        starts[t] = bsum[2 * blockIdx.x] + a - xa;
+        slot_of[t] = xb ? bsum[2 * blockIdx.x + 1] + b - xb : -1;
+    }
+}
+
+// Tile segments of the sorted order, and the compacted list of non-empty
+// tiles (increasing tile order; tile t at slot_of[t]).
 __global__ __launch_bounds__(256) void k_tile_segments(const int* __restrict__ starts,
-                                                       const int* __restrict__ totals, int ntiles,
+                                                       const int* __restrict__ totals,
+                                                       const int* __restrict__ slot_of, int ntiles,
                                                        int* __restrict__ seg_begin,
                                                        int* __restrict__ seg_end,
-                                                       unsigned char* __restrict__ nonempty) {
+                                                       int* __restrict__ tile_list) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntiles) return;
     seg_begin[t] = starts[t];
     seg_end[t] = starts[t] + totals[t];
-    nonempty[t] = totals[t] > 0 ? 1 : 0;
-}
-
-// slot_of[t]: the position of tile t in the compacted tile list (the tile
-// partials' slot), -1 for an empty tile.  The list holds the non-empty tiles
-// in increasing order, so tile t's position is the number of non-empty tiles
-// below it; one thread per tile finds it by binary search in the list.
-__global__ __launch_bounds__(256) void k_tile_slots(const unsigned char* __restrict__ nonempty,
-                                                    const int* __restrict__ tile_list,
-                                                    const int* __restrict__ count_ptr,
-                                                    int* __restrict__ slot_of, int ntiles) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles) return;
-    if (!nonempty[t]) {
-        slot_of[t] = -1;
-        return;
-    }
-    int lo = 0, hi = *count_ptr;  // tile_list[lo .. hi) is sorted; t is in it
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (tile_list[mid] <= t) lo = mid;
-        else hi = mid;
-    }
-    slot_of[t] = lo;
+    if (slot_of[t] >= 0) tile_list[slot_of[t]] = t;
 }
 
 // Destination of every particle in the sorted order: order[pos] = i and the
@@ -231,7 +312,7 @@ struct SortState {
     TileGeom tg{};
     // tile segments of the sorted order (valid after the first sort)
     int *seg_begin = nullptr, *seg_end = nullptr, *tile_list = nullptr, *tile_count = nullptr;
-    unsigned char* nonempty = nullptr;
+    int* scan_sums = nullptr;  // (particles, non-empty tiles) per scan block
     bool tiles_valid = false;
     unsigned *keys = nullptr, *keys_alt = nullptr, *count = nullptr;
     int *idx = nullptr, *idx_alt = nullptr;  // per-particle rank; the sorted order
@@ -240,8 +321,6 @@ struct SortState {
     int *offs = nullptr, *ovf = nullptr, *totals = nullptr, *starts = nullptr;
     int* rounds = nullptr;  // per tile: the largest particle count of one base
     int* slot_of = nullptr;  // per tile: its place in tile_list, -1 when empty
-    void* temp = nullptr;
-    size_t temp_bytes = 0;
     // Alternate buffers, one per permuted stream in for_each_stream order;
     // they swap roles with the particle streams on every permutation, so the
     // caller allocates and frees them together with the particle streams.
@@ -281,7 +360,8 @@ struct SortState {
         if (e == hipSuccess) e = hipMalloc((void**)&seg_end, sizeof(int) * tg.ntiles);
         if (e == hipSuccess) e = hipMalloc((void**)&tile_list, sizeof(int) * tg.ntiles);
         if (e == hipSuccess) e = hipMalloc((void**)&tile_count, sizeof(int));
-        if (e == hipSuccess) e = hipMalloc((void**)&nonempty, tg.ntiles);
+        if (e == hipSuccess)
+            e = hipMalloc((void**)&scan_sums, sizeof(int) * 2 * ((tg.ntiles + kScanBlock - 1) / kScanBlock));
         if (e == hipSuccess) e = hipMalloc((void**)&keys_alt, sizeof(unsigned) * np);
         if (e == hipSuccess) e = hipMalloc((void**)&idx, sizeof(int) * np);
         if (e == hipSuccess) e = hipMalloc((void**)&idx_alt, sizeof(int) * np);
@@ -297,33 +377,20 @@ struct SortState {
             error = std::string("hipMalloc: ") + hipGetErrorString(e);
             return MPM_E_NOMEM;
         }
-        // one workspace for the scan of the tile sizes and the tile-list compaction
-        size_t scan_bytes = 0, sel_bytes = 0;
-        e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, totals, starts, tg.ntiles);
-        if (e == hipSuccess)
-            e = hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, hipcub::CountingInputIterator<int>(0),
-                                              nonempty, tile_list, tile_count, tg.ntiles);
-        temp_bytes = scan_bytes > sel_bytes ? scan_bytes : sel_bytes;
-        if (e == hipSuccess) e = hipMalloc(&temp, temp_bytes ? temp_bytes : 1);
-        if (e != hipSuccess) {
-            error = std::string("sort workspace: ") + hipGetErrorString(e);
-            return MPM_E_NOMEM;
-        }
         return MPM_OK;
     }
 
     void release() {
-        void* ps[] = {keys, keys_alt, idx, idx_alt, temp, seg_begin, seg_end, tile_list,
-                      tile_count, nonempty, count, masks, offs, ovf, totals, starts, rounds, slot_of};
+        void* ps[] = {keys, keys_alt, idx, idx_alt, scan_sums, seg_begin, seg_end, tile_list,
+                      tile_count, count, masks, offs, ovf, totals, starts, rounds, slot_of};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         keys = keys_alt = count = nullptr;
         idx = idx_alt = nullptr;
         masks = nullptr;
         offs = ovf = totals = starts = rounds = slot_of = nullptr;
-        temp = nullptr;
+        scan_sums = nullptr;
         seg_begin = seg_end = tile_list = tile_count = nullptr;
-        nonempty = nullptr;
     }
 
 
@@ -353,7 +420,6 @@ struct SortState {
         if (!enabled) return MPM_OK;
         const unsigned nb = unsigned((np + 255) / 256);
         const int n = int(np);
-        hipError_t lib = hipSuccess;  // hipcub's own status (its launches are checked by launch())
         int rc = c->launch(6, [&] {
             k_fill_u32<<<unsigned((nbases + 255) / 256), 256, 0, s>>>(count, nbases);
             const double* x2 = dim == 3 ? pp.x[2] : pp.x[1];
@@ -366,12 +432,14 @@ struct SortState {
                 else k_sort_keys<2, 2><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
                 k_tile_layout<2><<<unsigned(tg.ntiles), 64, 0, s>>>(kg, count, masks, offs, ovf, totals, rounds, tg.ntiles);
             }
-            size_t bytes = temp_bytes;
-            lib = hipcub::DeviceScan::ExclusiveSum(temp, bytes, totals, starts, tg.ntiles, s);
+            const int nsb = int((tg.ntiles + kScanBlock - 1) / kScanBlock);
+            k_tile_scan_sums<<<unsigned(nsb), kScanBlock, 0, s>>>(totals, int(tg.ntiles), scan_sums);
+            k_tile_scan_blocks<<<1, kScanBlock, 0, s>>>(scan_sums, nsb, tile_count);
+            k_tile_scan_apply<<<unsigned(nsb), kScanBlock, 0, s>>>(totals, int(tg.ntiles), scan_sums, starts,
+                                                                   slot_of);
             k_sort_dest<<<nb, 256, 0, s>>>(keys, idx, starts, masks, offs, ovf, idx_alt, keys_alt, n);
         });
         if (rc != MPM_OK) return rc;
-        if (lib != hipSuccess) return c->hip_fail(lib, "tile scan");
         const int* order = idx_alt;
         PermuteStreams ps{};
         for_each_stream(pp, dim, [&](auto*& ptr) {
@@ -395,16 +463,8 @@ struct SortState {
         // tile segments of the new order and the ordered list of non-empty tiles
         rc = c->launch(6, [&] {
             const unsigned tb = unsigned((tg.ntiles + 255) / 256);
-            k_tile_segments<<<tb, 256, 0, s>>>(starts, totals, tg.ntiles, seg_begin, seg_end, nonempty);
-            size_t bytes = temp_bytes;
-            lib = hipcub::DeviceSelect::Flagged(temp, bytes, hipcub::CountingInputIterator<int>(0),
-                                                nonempty, tile_list, tile_count, tg.ntiles, s);
-        });
-        if (rc != MPM_OK) return rc;
-        if (lib != hipSuccess) return c->hip_fail(lib, "tile list compaction");
-        rc = c->launch(6, [&] {
-            const unsigned tb = unsigned((tg.ntiles + 255) / 256);
-            k_tile_slots<<<tb, 256, 0, s>>>(nonempty, tile_list, tile_count, slot_of, tg.ntiles);
+            k_tile_segments<<<tb, 256, 0, s>>>(starts, totals, slot_of, int(tg.ntiles), seg_begin, seg_end,
+                                               tile_list);
         });
         if (rc != MPM_OK) return rc;
         tiles_valid = true;

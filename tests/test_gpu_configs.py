@@ -154,9 +154,10 @@ def test_slope_20m_k0_and_runout(mpm):
         sa = m.step_args(ra)
         done = 0
         while done < nrun:
-            k = min(2_000, nrun - done)
+            k = min(1_000, nrun - done)
             Ti = m.run(sa, dt, Ti, k, graph=True)
             done += k
+            m.sync()  # raises if a particle left the grid
             if done == 2_000:
                 early = m.download()
             print(f"runout: {done} of {nrun} steps, t = {Ti:.2f} s", flush=True)
