@@ -292,7 +292,7 @@ def _bilinear(field, x0, dx, y0, dy, x, y):
 def slope_3d(*, h=0.5, length=400.0, width=200.0, angle_deg=35.0, depth=10.0,
              footprint=(250.0, 125.0), footprint_x0=20.0, corr_len=20.0, grf_std=2.0,
              phi_deg_range=(25.0, 35.0), cohesion_median=5.0e3, cohesion_sigma_ln=0.5,
-             rho=2000.0, E_mod=10.0e6, nu=0.3, seed=7, headroom=60.0, pad=2, eps="FP32",
+             rho=2000.0, E_mod=10.0e6, nu=0.3, seed=7, headroom=25.0, pad=2, eps="FP32",
              cfl=0.2, k0_steps=2000, zeta_k0=0.1, runout_time=30.0, device="HIP"):
     """Synthetic 3D slope failure in a slope-aligned frame (x downslope, y
     across, z normal to the mean plane), Mohr-Coulomb.
@@ -309,10 +309,10 @@ def slope_3d(*, h=0.5, length=400.0, width=200.0, angle_deg=35.0, depth=10.0,
     elastic steps with a gravity ramp, Te = k0_steps dt); `runout_args` in the
     layout is the undamped runout phase.  The defaults give 20,000,000
     particles (250 x 125 m footprint, 10 m deep, h = 0.5 m).  The grid reaches
-    `headroom` above the highest bed point plus the depth: the mass that runs
-    into the x = length wall piles up to about 45 m above the bed within the
-    30 s runout (measured at h = 2 m), so 60 m keeps every particle's stencil
-    inside the grid.
+    `headroom` above the highest bed point plus the depth.  The mass reaches
+    the x = length wall after about 12 s of runout and piles up there to about
+    45 m above the bed (measured at h = 2 m): a runout past that needs
+    headroom >= 60.
 
     Returns (args, grid, mp, attr, bc, layout)."""
     rng = np.random.default_rng(seed + 1)
