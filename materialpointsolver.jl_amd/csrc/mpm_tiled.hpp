@@ -810,8 +810,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LAY ? 3 : 4)
 // then the strain / constitutive / MVL / position commit of g2p2_commit.
 // Four wavefronts share the window of a tile.
 // ---------------------------------------------------------------------------
+#ifndef MPM_G2P2_WAVES
+#define MPM_G2P2_WAVES 1
+#endif
+#ifndef MPM_G2P1_WAVES
+#define MPM_G2P1_WAVES 1
+#endif
 template <int DIM, int S, int TSH, typename R, int CM>
-__global__ __launch_bounds__(256) void k_g2p2_tiled(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPM_G2P2_WAVES))) void k_g2p2_tiled(
     GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, const Material<R>* __restrict__ mats,
     StepArgs<R> sa, const int* __restrict__ tile_list, const int* __restrict__ seg_begin,
     const int* __restrict__ seg_end, const int* __restrict__ count_ptr, int* flags) {
@@ -895,7 +901,7 @@ __global__ __launch_bounds__(256) void k_g2p2_tiled(
 // VS: whether vs is gathered.  The :a form with PIC = 0 multiplies its sum by
 // zero, so that window drops a third of its fields (the default FLIP = 1).
 template <int DIM, int S, int TSH, typename R, bool VS>
-__global__ __launch_bounds__(256) void k_g2p1_tiled(GridGeom g, TileGeom tg, PartPtrs<R> pp,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPM_G2P1_WAVES))) void k_g2p1_tiled(GridGeom g, TileGeom tg, PartPtrs<R> pp,
                                                    GridPtrs<R> gp, StepArgs<R> sa,
                                                    const int* __restrict__ tile_list,
                                                    const int* __restrict__ seg_begin,
