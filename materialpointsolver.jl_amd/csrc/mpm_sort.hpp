@@ -41,6 +41,10 @@ struct SortKeyGeom {
     int tx, ty;  // tiles per axis (x, y)
 };
 
+// Sort key of a frozen particle; valid keys stay below it (the tile count is
+// checked against the 32-bit key at set-up).
+constexpr unsigned kFrozenKey = 0xffffffffu;
+
 template <int DIM, int S>
 __global__ __launch_bounds__(256) void k_sort_keys(GridGeom g, SortKeyGeom kg,
                                                    const double* __restrict__ x0,
@@ -48,17 +52,29 @@ __global__ __launch_bounds__(256) void k_sort_keys(GridGeom g, SortKeyGeom kg,
                                                    const double* __restrict__ x2,
                                                    unsigned* __restrict__ count,
                                                    unsigned* __restrict__ keys,
-                                                   int* __restrict__ rank_out, int np) {
+                                                   int* __restrict__ rank_out, int np,
+                                                   int* __restrict__ flags) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const double* xs[3] = {x0, x1, x2};
     int b[3] = {0, 0, 0};
+    bool inside = true, finite = true;
 #pragma unroll
     for (int a = 0; a < DIM; ++a) {
-        double f = stencil_base_f<S>(a, (xs[a][i] - g.org[a]) * g.hinv[a]);
-        if (!(f >= 0.0)) f = 0.0;  // also maps NaN to 0
-        if (f > double(kg.nb[a] - 1)) f = double(kg.nb[a] - 1);
-        b[a] = int(f);
+        const double u = (xs[a][i] - g.org[a]) * g.hinv[a];
+        const double f = stencil_base_f<S>(a, u);
+        finite = finite && isfinite(u);
+        if (!(f >= 0.0 && f <= double(kg.nb[a] - 1))) inside = false;  // also NaN
+        else b[a] = int(f);
+    }
+    if (!inside) {
+        // a frozen particle (its stencil left the grid, or x is not finite):
+        // placed after every tile by k_sort_dest, in no tile segment, and
+        // reported as the kernels that skip it report it
+        keys[i] = kFrozenKey;
+        rank_out[i] = 0;
+        atomicOr(flags, finite ? int(FLAG_RANGE) : int(FLAG_NONFINITE));
+        return;
     }
     const unsigned lin = unsigned(b[0]) + unsigned(kg.nb[0]) * (unsigned(b[1]) + unsigned(kg.nb[1]) * unsigned(b[2]));
     const unsigned rank = atomicAdd(&count[lin], 1u);
@@ -251,11 +267,21 @@ __global__ __launch_bounds__(256) void k_sort_dest(const unsigned* __restrict__ 
                                                    const unsigned long long* __restrict__ masks,
                                                    const int* __restrict__ offs,
                                                    const int* __restrict__ ovf,
+                                                   const int* __restrict__ totals, int ntiles,
+                                                   unsigned* __restrict__ frozen,
                                                    int* __restrict__ order,
                                                    unsigned* __restrict__ keys_sorted, int np) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const unsigned key = keys[i];
+    if (key == kFrozenKey) {
+        // after the last tile, in arrival order (frozen particles take part in
+        // no computation, so their order does not matter)
+        const int pos = starts[ntiles - 1] + totals[ntiles - 1] + int(atomicAdd(frozen, 1u));
+        order[pos] = i;
+        keys_sorted[pos] = key;
+        return;
+    }
     const long long t = key >> kTileShift;
     const int l = int(key & ((1u << kLocalBits) - 1));
     const int r = rank[i];
@@ -355,7 +381,8 @@ struct SortState {
         tg.tz = tz;
         tg.ntiles = kg.tx * kg.ty * tz;
         hipError_t e = hipMalloc((void**)&keys, sizeof(unsigned) * np);
-        if (e == hipSuccess) e = hipMalloc((void**)&count, sizeof(unsigned) * nbases);
+        // per-base counts, then the frozen-particle counter
+        if (e == hipSuccess) e = hipMalloc((void**)&count, sizeof(unsigned) * (nbases + 1));
         if (e == hipSuccess) e = hipMalloc((void**)&seg_begin, sizeof(int) * tg.ntiles);
         if (e == hipSuccess) e = hipMalloc((void**)&seg_end, sizeof(int) * tg.ntiles);
         if (e == hipSuccess) e = hipMalloc((void**)&tile_list, sizeof(int) * tg.ntiles);
@@ -421,15 +448,15 @@ struct SortState {
         const unsigned nb = unsigned((np + 255) / 256);
         const int n = int(np);
         int rc = c->launch(6, [&] {
-            k_fill_u32<<<unsigned((nbases + 255) / 256), 256, 0, s>>>(count, nbases);
+            k_fill_u32<<<unsigned((nbases + 1 + 255) / 256), 256, 0, s>>>(count, nbases + 1);
             const double* x2 = dim == 3 ? pp.x[2] : pp.x[1];
             if (dim == 3) {
-                if (S == 3) k_sort_keys<3, 3><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
-                else k_sort_keys<3, 2><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
+                if (S == 3) k_sort_keys<3, 3><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n, c->d_flags);
+                else k_sort_keys<3, 2><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n, c->d_flags);
                 k_tile_layout<3><<<unsigned(tg.ntiles), 64, 0, s>>>(kg, count, masks, offs, ovf, totals, rounds, tg.ntiles);
             } else {
-                if (S == 3) k_sort_keys<2, 3><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
-                else k_sort_keys<2, 2><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n);
+                if (S == 3) k_sort_keys<2, 3><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n, c->d_flags);
+                else k_sort_keys<2, 2><<<nb, 256, 0, s>>>(geom, kg, pp.x[0], pp.x[1], x2, count, keys, idx, n, c->d_flags);
                 k_tile_layout<2><<<unsigned(tg.ntiles), 64, 0, s>>>(kg, count, masks, offs, ovf, totals, rounds, tg.ntiles);
             }
             const int nsb = int((tg.ntiles + kScanBlock - 1) / kScanBlock);
@@ -437,7 +464,8 @@ struct SortState {
             k_tile_scan_blocks<<<1, kScanBlock, 0, s>>>(scan_sums, nsb, tile_count);
             k_tile_scan_apply<<<unsigned(nsb), kScanBlock, 0, s>>>(totals, int(tg.ntiles), scan_sums, starts,
                                                                    slot_of);
-            k_sort_dest<<<nb, 256, 0, s>>>(keys, idx, starts, masks, offs, ovf, idx_alt, keys_alt, n);
+            k_sort_dest<<<nb, 256, 0, s>>>(keys, idx, starts, masks, offs, ovf, totals, int(tg.ntiles), count + nbases,
+                                          idx_alt, keys_alt, n);
         });
         if (rc != MPM_OK) return rc;
         const int* order = idx_alt;

@@ -810,14 +810,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LAY ? 3 : 4)
 // then the strain / constitutive / MVL / position commit of g2p2_commit.
 // Four wavefronts share the window of a tile.
 // ---------------------------------------------------------------------------
-#ifndef MPM_G2P2_WAVES
-#define MPM_G2P2_WAVES 1
-#endif
-#ifndef MPM_G2P1_WAVES
-#define MPM_G2P1_WAVES 1
-#endif
+// Wavefronts per SIMD the FP32 gathers are compiled for.  Left to itself the
+// compiler gives G2P #2 (DP) 189 registers (2 wavefronts per SIMD, its commit
+// loads all in flight) and the re-map gather 130 (3); held to 3 wavefronts
+// (168 registers, two spilled in G2P #2) they measured 120.8 -> 105.1 us and
+// 76.9 -> 66.7 us at step 40 of the 3D column, 174.8 -> 150.6 and 115.0 -> 97.2
+// at step 5,040; at 4 G2P #2 spills 50 registers and takes twice as long.  The
+// FP64 forms keep the compiler's choice (3 wavefronts would spill 50-140).
+constexpr int kGatherWaves = 3;
+template <typename R>
+constexpr int gather_waves() { return sizeof(R) == 4 ? kGatherWaves : 1; }
+
 template <int DIM, int S, int TSH, typename R, int CM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPM_G2P2_WAVES))) void k_g2p2_tiled(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_waves<R>()))) void k_g2p2_tiled(
     GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, const Material<R>* __restrict__ mats,
     StepArgs<R> sa, const int* __restrict__ tile_list, const int* __restrict__ seg_begin,
     const int* __restrict__ seg_end, const int* __restrict__ count_ptr, int* flags) {
@@ -901,7 +906,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPM_G2P2_WA
 // VS: whether vs is gathered.  The :a form with PIC = 0 multiplies its sum by
 // zero, so that window drops a third of its fields (the default FLIP = 1).
 template <int DIM, int S, int TSH, typename R, bool VS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MPM_G2P1_WAVES))) void k_g2p1_tiled(GridGeom g, TileGeom tg, PartPtrs<R> pp,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_waves<R>()))) void k_g2p1_tiled(GridGeom g, TileGeom tg, PartPtrs<R> pp,
                                                    GridPtrs<R> gp, StepArgs<R> sa,
                                                    const int* __restrict__ tile_list,
                                                    const int* __restrict__ seg_begin,

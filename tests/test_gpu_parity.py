@@ -385,3 +385,52 @@ def test_fp32_tiled_kernels_match_plain_kernels(mpm, dim, basis):
     finally:
         tiled.close()
         plain.close()
+
+
+# Particles whose stencil leaves the grid are frozen: every kernel skips them
+# and mpm_sync reports them.  The sort places them after every tile, outside
+# every tile segment, so they cost the tile kernels nothing; the tiled run must
+# give the plain run's state, frozen particles included.  A small FP64 column
+# without any boundary condition and a downward initial velocity falls out
+# through the bottom of the grid part by part.
+def test_frozen_particles_tiled_and_plain(mpm):
+    states = {}
+    for sort in (0, 5):
+        args, grid, mp, attr, bc = _c3(mpm)
+        for k in ("vx_s_idx", "vy_s_idx", "vz_s_idx"):
+            getattr(bc, k)[:] = 0
+        mp.vs[:, 2] = -5.0
+        print(f"case: frozen particles, 3D FP64 np={mp.np} sort={sort}", flush=True)
+        model = mpm.HIPModel(grid, mp, attr, bc, sort_interval=sort)
+        try:
+            sargs = model.step_args(args)
+            dt = float(args.ΔT)
+            Ti = model.run(sargs, dt, 0.0, 60)
+            with pytest.raises(mpm._capi.MPMError, match="left the grid"):
+                model.sync()
+            mid = model.download()
+            model.run(sargs, dt, Ti, 20)
+            with pytest.raises(mpm._capi.MPMError):
+                model.sync()
+            end = model.download()
+        finally:
+            model.close()
+        # the uGIMP stencil base floor(u - 1/2) is below the grid for z < z1 + dz / 2
+        gone = mid["xi"][:, 2] < float(grid.z1) + 0.5 * float(grid.dz)
+        # frozen particles keep their positions
+        assert np.array_equal(end["xi"][gone], mid["xi"][gone])
+        states[sort] = (gone, end)
+        print(f"  {gone.sum()} of {mp.np} particles frozen after 60 steps", flush=True)
+    (g0, e0), (g5, e5) = states[0], states[5]
+    assert 0 < g0.sum() < len(g0) and np.array_equal(g0, g5)
+    for key in ("xi", "vs"):
+        d = np.linalg.norm(e5[key] - e0[key]) / np.linalg.norm(e0[key])
+        print(f"  tiled vs plain {key}: rel L2 {d:.1e}", flush=True)
+        assert d <= 1e-9
+    # in free fall the stresses stay at rounding level: compare them against the
+    # column's own scale rho g H instead
+    scale = 2650.0 * 9.8 * 0.08
+    ds = np.abs(e5["sigma"] - e0["sigma"]).max()
+    print(f"  tiled vs plain sigma: max |diff| {ds:.1e} Pa (largest |sigma| "
+          f"{np.abs(e0['sigma']).max():.1e} Pa, rho g H {scale:.0f} Pa)", flush=True)
+    assert ds <= 1e-9 * scale
