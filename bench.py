@@ -97,10 +97,10 @@ def parse():
     ap.add_argument("--slope-k0", type=int, default=400, help="damped K0 steps before the runout")
     ap.add_argument("--slope-steps", type=int, default=200, help="timed runout steps per form")
     ap.add_argument("--scatter", default=None,
-                    choices=["rounds", "rounds_layered", "owned", "owned_layers"],
+                    choices=["rounds", "rounds_layered", "owned", "owned_layers", "pairs"],
                     help="tile-scatter form (default: the library's)")
     ap.add_argument("--remap-scatter", default=None,
-                    choices=["rounds", "rounds_layered", "owned", "owned_layers"],
+                    choices=["rounds", "rounds_layered", "owned", "owned_layers", "pairs"],
                     help="the re-map momentum scatter's form (default: the library's)")
     ap.add_argument("--fused-remap", type=int, default=None, choices=[0, 1],
                     help="one-pass MUSL re-map (default: the library's)")
@@ -114,11 +114,11 @@ def parse():
 # (a = b + s c over three float4 arrays of TRIAD_BYTES each: reads 2x, writes 1x).
 TRIAD_BYTES = 256 << 20
 KERNEL_RE = {
-    "k_p2g": r"mpm::k_p2g<|mpm::k_scatter_(tiled|owned)<\d, \d, \d, \w+, 0[,>]",
+    "k_p2g": r"mpm::k_p2g<|mpm::k_scatter_(tiled|owned)<\d, \d, \d, \w+, 0[,>]|mpm::k_scatter_pairs<\d, \d, 0>",
     "k_grid1": r"mpm::k_grid1<",
     "k_g2p1_p2g2": (r"mpm::k_g2p1(_p2g2)?(_tiled)?<|mpm::k_scatter_tiled<\d, \d, \d, \w+, [23][,>]"
                     r"|mpm::k_remap_owned<"),
-    "k_p2g2": r"mpm::k_scatter_(tiled|owned)<\d, \d, \d, \w+, 1[,>]",
+    "k_p2g2": r"mpm::k_scatter_(tiled|owned)<\d, \d, \d, \w+, 1[,>]|mpm::k_scatter_pairs<\d, \d, 1>",
     "k_grid2": r"mpm::k_grid2<",
     "k_g2p2": r"mpm::k_g2p2(_tiled)?<",
 }
@@ -404,7 +404,7 @@ def main():
             "grid_nodes": int(grid.ni),
             "active_nodes": n_active,
             "sort_interval": a.sort_interval,
-            "kernel_variant": {"scatter": a.scatter or "rounds (default)",
+            "kernel_variant": {"scatter": a.scatter or "pairs (default in FP32 3D)",
                                "remap_scatter": a.remap_scatter or "owned (default in FP32 3D)",
                                "fused_remap": bool(a.fused_remap) if a.fused_remap is not None
                                else "off (default)",

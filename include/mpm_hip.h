@@ -169,9 +169,11 @@ int mpm_run_graph(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int6
  *                         (default).  Fewer event pairs per step keep the
  *                         profiled steps closer to unprofiled ones.
  *  MPM_OPT_SCATTER:       form of the tile scatter of P2G #1 (and of the MUSL
- *                         re-map's, see MPM_OPT_REMAP_SCATTER):
- *                         MPM_SCATTER_ROUNDS (default): rank rounds of
- *                         distinct bases adding into an LDS window;
+ *                         re-map's, see MPM_OPT_REMAP_SCATTER); -1 (default):
+ *                         MPM_SCATTER_PAIRS in FP32 3D uGIMP, elsewhere
+ *                         MPM_SCATTER_ROUNDS:
+ *                         MPM_SCATTER_ROUNDS: rank rounds of distinct bases
+ *                         adding into an LDS window;
  *                         MPM_SCATTER_OWNED: each lane owns one stencil base
  *                         of a tile and sums its particles in registers (3D
  *                         P2G: mass and momentum, then force);
@@ -181,7 +183,11 @@ int mpm_run_graph(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int6
  *                         group adding into three LDS windows, one per
  *                         stencil z offset, so each round's LDS updates form
  *                         three independent chains (FP32 3D uGIMP; elsewhere
- *                         it runs as MPM_SCATTER_ROUNDS).
+ *                         it runs as MPM_SCATTER_ROUNDS);
+ *                         MPM_SCATTER_PAIRS: pair rounds: a lane takes two
+ *                         particles of one stencil base and does one LDS
+ *                         read-add-write per slot for both (FP32 3D uGIMP;
+ *                         elsewhere it runs as MPM_SCATTER_OWNED).
  *  MPM_OPT_FUSED_REMAP:   0 (default): with the tile kernels, the MUSL re-map
  *                         is a gather kernel, then the momentum scatter;
  *                         1: it gathers, moves the particles and scatters the
@@ -195,11 +201,11 @@ int mpm_run_graph(mpm_ctx *ctx, const mpm_args *args, double dt, double Ti, int6
  *  MPM_OPT_REMAP_SCATTER: form of the MUSL re-map's momentum scatter (an
  *                         MPM_SCATTER_* value); -1 (default): base-owned
  *                         (MPM_SCATTER_OWNED) in FP32 3D uGIMP, elsewhere the
- *                         form of MPM_OPT_SCATTER. */
+ *                         form P2G #1 runs in. */
 enum { MPM_OPT_SORT_INTERVAL = 0, MPM_OPT_TILES = 1, MPM_OPT_PROFILE_MASK = 2, MPM_OPT_SCATTER = 3,
        MPM_OPT_FUSED_REMAP = 4, MPM_OPT_TILE_PARTIALS = 5, MPM_OPT_REMAP_SCATTER = 6 };
 enum { MPM_SCATTER_ROUNDS = 0, MPM_SCATTER_OWNED = 1, MPM_SCATTER_OWNED_LAYERS = 2,
-       MPM_SCATTER_ROUNDS_LAYERED = 3 };
+       MPM_SCATTER_ROUNDS_LAYERED = 3, MPM_SCATTER_PAIRS = 4 };
 int mpm_set_option(mpm_ctx *ctx, int option, int64_t value);
 
 /* Wait for all work on the context's stream; reports range/finite flags the

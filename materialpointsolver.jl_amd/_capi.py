@@ -26,9 +26,10 @@ MPM_VA_A, MPM_VA_V = 0, 1
 MPM_OPT_SORT_INTERVAL, MPM_OPT_TILES, MPM_OPT_PROFILE_MASK, MPM_OPT_SCATTER = 0, 1, 2, 3
 MPM_OPT_FUSED_REMAP, MPM_OPT_TILE_PARTIALS, MPM_OPT_REMAP_SCATTER = 4, 5, 6
 MPM_SCATTER_ROUNDS, MPM_SCATTER_OWNED, MPM_SCATTER_OWNED_LAYERS = 0, 1, 2
-MPM_SCATTER_ROUNDS_LAYERED = 3
+MPM_SCATTER_ROUNDS_LAYERED, MPM_SCATTER_PAIRS = 3, 4
 SCATTER = {"rounds": MPM_SCATTER_ROUNDS, "owned": MPM_SCATTER_OWNED,
-           "owned_layers": MPM_SCATTER_OWNED_LAYERS, "rounds_layered": MPM_SCATTER_ROUNDS_LAYERED}
+           "owned_layers": MPM_SCATTER_OWNED_LAYERS, "rounds_layered": MPM_SCATTER_ROUNDS_LAYERED,
+           "pairs": MPM_SCATTER_PAIRS}
 CONSTITUTIVE = {"linearelastic": 0, "druckerprager": 1, "mohrcoulomb": 2}
 
 

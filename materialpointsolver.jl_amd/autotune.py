@@ -11,7 +11,10 @@ re-map):
   wavefront (12.6 KB of LDS per wavefront);
 * ``"rounds_layered"``: rank rounds of one group adding into three layer
   windows, one per stencil z offset (the same 12.6 KB): three independent
-  chains of LDS updates per round (FP32 3D uGIMP; elsewhere ``"rounds"``).
+  chains of LDS updates per round (FP32 3D uGIMP; elsewhere ``"rounds"``);
+* ``"pairs"``: pair rounds: a lane takes two particles of one stencil base
+  and does one LDS read-add-write per slot for both (half the window traffic
+  of ``"rounds"``; FP32 3D uGIMP, elsewhere ``"owned"``).
 
 The tile shape (4x4x4 bases in 3D, 8x8 in 2D) and the window size (no slack)
 are compile-time.  The MUSL re-map's momentum scatter can take its own form
@@ -26,7 +29,7 @@ from __future__ import annotations
 
 import time
 
-SCATTER_FORMS = ("owned", "owned_layers", "rounds", "rounds_layered")
+SCATTER_FORMS = ("owned", "owned_layers", "rounds", "rounds_layered", "pairs")
 
 
 def _set_scatter(model, sc):

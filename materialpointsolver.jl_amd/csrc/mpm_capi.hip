@@ -97,7 +97,7 @@ struct mpm_ctx {
     bool prof_on = false;
     uint64_t prof_mask = 0;  // kernel ids to time; 0 = all
     bool use_tiles = true;  // LDS-binned scatters when the particles are tile-sorted
-    int scatter = MPM_SCATTER_ROUNDS;  // tile scatter form (MPM_OPT_SCATTER)
+    int scatter = -1;                  // tile scatter form, -1: automatic (MPM_OPT_SCATTER)
     int remap_scatter = -1;            // re-map scatter form, -1: automatic (MPM_OPT_REMAP_SCATTER)
     bool fused_remap = false;         // one-pass MUSL re-map (MPM_OPT_FUSED_REMAP)
     bool tile_partials = true;        // tile scatters store windows (MPM_OPT_TILE_PARTIALS)
@@ -580,13 +580,18 @@ struct Impl : ImplBase {
         // layer windows for the rank rounds: FP32 3D uGIMP (elsewhere the
         // two-window rounds run)
         constexpr bool kLayerable = DIM == 3 && S == 3 && sizeof(R) == 4;
-        // scatter forms of P2G #1 and of the re-map's momentum scatter; the
-        // re-map's follows P2G #1's unless set (MPM_OPT_REMAP_SCATTER), and
-        // is base-owned by default in FP32 3D uGIMP (measured fastest there)
-        const int form1 = c->scatter;
+        // scatter forms of P2G #1 and of the re-map's momentum scatter; by
+        // default (measured fastest) FP32 3D uGIMP runs P2G #1 in pair rounds
+        // and the re-map scatter base-owned, everything else in rank rounds;
+        // the re-map's form follows P2G #1's unless set (MPM_OPT_REMAP_SCATTER)
+        const int form1 = c->scatter >= 0 ? c->scatter
+                                          : (kLayerable ? int(MPM_SCATTER_PAIRS) : int(MPM_SCATTER_ROUNDS));
         const int form2 = c->remap_scatter >= 0 ? c->remap_scatter
-                                                : (kLayerable ? int(MPM_SCATTER_OWNED) : c->scatter);
-        auto is_owned = [](int f) { return f == MPM_SCATTER_OWNED || f == MPM_SCATTER_OWNED_LAYERS; };
+                                                : (kLayerable ? int(MPM_SCATTER_OWNED) : form1);
+        // MPM_SCATTER_PAIRS is FP32 3D uGIMP only (elsewhere it runs as MPM_SCATTER_OWNED)
+        auto is_owned = [](int f) {
+            return f == MPM_SCATTER_OWNED || f == MPM_SCATTER_OWNED_LAYERS || f == MPM_SCATTER_PAIRS;
+        };
         const bool owned = is_owned(form1), owned2 = is_owned(form2);
         const bool layered = kLayerable && form1 == MPM_SCATTER_ROUNDS_LAYERED;
         const bool layered2 = kLayerable && form2 == MPM_SCATTER_ROUNDS_LAYERED;
@@ -619,6 +624,14 @@ struct Impl : ImplBase {
                 DIM == 3 && (sizeof(R) == 8 || (MODE == 0 ? form1 : form2) == MPM_SCATTER_OWNED_LAYERS);
             constexpr int SP_FIELDS = MODE == 0 ? 1 : 0;
             constexpr bool F64 = sizeof(R) == 8;
+            if constexpr (kLayerable) {
+                if ((MODE == 0 ? form1 : form2) == MPM_SCATTER_PAIRS) {
+                    k_scatter_pairs<S, TSH, MODE><<<sort.tiled_grid(), 64, 0, s>>>(
+                        geom, sort.tg, pp, gp, Gv, sort.masks, sort.offs, sort.ovf, sort.rounds,
+                        sort.tile_list, sort.seg_begin, sort.seg_end, sort.tile_count, fl, tbp);
+                    return;
+                }
+            }
             if (DIM == 3 && (F64 || layers))
                 k_scatter_owned<DIM, S, TSH, R, MODE, DIM == 3 ? 2 : 0><<<sort.tiled_grid(), 64, 0, s>>>(
                     geom, sort.tg, pp, gp, Gv, sort.masks, sort.offs, sort.ovf, sort.rounds,
@@ -1321,15 +1334,17 @@ int mpm_set_option(mpm_ctx* c, int option, int64_t value) {
     } else if (option == MPM_OPT_TILE_PARTIALS) {
         c->tile_partials = value != 0;
     } else if (option == MPM_OPT_SCATTER) {
-        if (value != MPM_SCATTER_ROUNDS && value != MPM_SCATTER_OWNED && value != MPM_SCATTER_OWNED_LAYERS &&
-            value != MPM_SCATTER_ROUNDS_LAYERED)
-            return c->fail(MPM_E_ARG, "mpm_set_option: scatter form is MPM_SCATTER_ROUNDS, "
-                                      "MPM_SCATTER_OWNED, MPM_SCATTER_OWNED_LAYERS or "
-                                      "MPM_SCATTER_ROUNDS_LAYERED");
+        if (value != -1 && value != MPM_SCATTER_ROUNDS && value != MPM_SCATTER_OWNED &&
+            value != MPM_SCATTER_OWNED_LAYERS && value != MPM_SCATTER_ROUNDS_LAYERED &&
+            value != MPM_SCATTER_PAIRS)
+            return c->fail(MPM_E_ARG, "mpm_set_option: scatter form is -1 (automatic), "
+                                      "MPM_SCATTER_ROUNDS, MPM_SCATTER_OWNED, MPM_SCATTER_OWNED_LAYERS, "
+                                      "MPM_SCATTER_ROUNDS_LAYERED or MPM_SCATTER_PAIRS");
         c->scatter = int(value);
     } else if (option == MPM_OPT_REMAP_SCATTER) {
         if (value != -1 && value != MPM_SCATTER_ROUNDS && value != MPM_SCATTER_OWNED &&
-            value != MPM_SCATTER_OWNED_LAYERS && value != MPM_SCATTER_ROUNDS_LAYERED)
+            value != MPM_SCATTER_OWNED_LAYERS && value != MPM_SCATTER_ROUNDS_LAYERED &&
+            value != MPM_SCATTER_PAIRS)
             return c->fail(MPM_E_ARG, "mpm_set_option: re-map scatter form is -1 (automatic) or "
                                       "an MPM_SCATTER_* form");
         c->remap_scatter = int(value);

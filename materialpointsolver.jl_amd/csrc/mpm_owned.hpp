@@ -594,6 +594,346 @@ __global__ __launch_bounds__(64) void k_scatter_owned(
     raise_flag(flags, flag);
 }
 
+// FP32 uGIMP per-axis constants of make_stencil staged in LDS (kStencilK per
+// axis: h, 2h, 1/h, lp^2, c4, c1, lp/2, h + lp/2, h - lp/2, as ugimp3_f32 forms
+// them).  Read back with volatile loads at each use, they are not hoisted out
+// of the particle loop into registers the sums need.
+constexpr int kStencilK = 9;
+__device__ __forceinline__ void stage_stencil_consts(const GridGeom& g, lds_ptr<float> kc) {
+    const int t = threadIdx.x;
+    if (t < 3) {
+        const float h = float(g.h[t]), half = float(g.half[t]), lp = float(g.lp[t]);
+        lds_ptr<float> k = kc + t * kStencilK;
+        k[0] = h;
+        k[1] = 2.0f * h;
+        k[2] = float(g.hinv[t]);
+        k[3] = lp * lp;
+        k[4] = float(g.c4[t]);
+        k[5] = float(g.c1[t]);
+        k[6] = half;
+        k[7] = h + half;
+        k[8] = h - half;
+    }
+}
+
+// make_stencil's FP32 3D uGIMP path with the per-axis constants from LDS.
+__device__ __forceinline__ bool make_stencil3_lds(const GridGeom& g, const double* x,
+                                                  Stencil<3, 3, float>& st, int& flag,
+                                                  lds_ptr<float> kc) {
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double u = (x[a] - g.org[a]) * g.hinv[a];
+        if (!isfinite(u)) {
+            flag |= FLAG_NONFINITE;
+            return false;
+        }
+        const double fb = stencil_base_f<3>(a, u);
+        int b = 0;
+        if (fb < 0.0 || fb > double(g.nn[a] - 3)) ok = false;
+        else b = int(fb);
+        st.base[a] = b;
+        const float d0 = float(x[a] - (g.org[a] + fb * g.h[a]));
+        lds_ptr<float> k = kc + a * kStencilK;
+        const float h = k[0], h2 = k[1], hinv = k[2], lp2 = k[3], c4 = k[4], c1 = k[5],
+                    half = k[6], hph = k[7], hmh = k[8];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const float d = j == 0 ? d0 : (j == 1 ? d0 - h : d0 - h2);
+            const float ad = fabsf(d);
+            const float t = hph - ad;
+            const float sg = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+            const float Nc = 1.0f - (4.0f * d * d + lp2) * c4;
+            const float Nl = 1.0f - ad * hinv;
+            const float Nq = t * t * 0.5f * c1;
+            const float dNc = -8.0f * d * c4;
+            const float dNl = -sg * hinv;
+            const float dNq = -sg * t * c1;
+            const bool in_c = ad < half, in_l = ad < hmh, in_q = ad < hph;
+            st.N[a][j] = in_c ? Nc : (in_l ? Nl : (in_q ? Nq : 0.0f));
+            st.dN[a][j] = in_c ? dNc : (in_l ? dNl : (in_q ? dNq : 0.0f));
+        }
+    }
+    if (!ok) flag |= FLAG_RANGE;
+    return ok;
+}
+
+// ---------------------------------------------------------------------------
+// Pair rounds (FP32 3D uGIMP).  A lane takes two particles of one stencil
+// base, ranks 2k and 2k + 1 of the last sort, and per stencil slot adds both
+// contributions in registers, then does one read-add-write of the window node:
+// half the window traffic and adds of the rank rounds (one read-add-write per
+// particle and slot), with two particles' stencils in registers instead of the
+// base-owned 27 x 7 sums.  A round holds the bases of pair group k (the bases
+// with more than 2k particles, distinct, so their targets are distinct at
+// every slot) and, when both fit in 64 lanes, those of group k + 1 in a second
+// window (the two are summed before the flush).  Lane j of group k holds
+// particle start + off[2k] + j, of the j-th base of mask[2k] (found through a
+// 64-entry LDS table the base's lane writes), and its partner
+// start + off[2k + 1] + popcount(mask[2k + 1] & bases below) (mpm_sort.hpp's
+// layout).  Ranks from 30 on (a dual cell with more than 30 particles) are
+// taken with one lane per base.  A particle off its sort-time base goes to a
+// drift list and claims its base as in k_scatter_owned.  One wavefront per tile.
+// ---------------------------------------------------------------------------
+#ifndef MPM_PAIRS_WAVES
+#define MPM_PAIRS_WAVES 3
+#endif
+constexpr int kPairsWaves = MPM_PAIRS_WAVES;  // P2G #1's wavefronts per SIMD
+template <int S, int TSH, int MODE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODE == 0 ? kPairsWaves : 4))) void k_scatter_pairs(
+    GridGeom g, TileGeom tg, PartPtrs<float> pp, GridPtrs<float> gp, Gravity<float> G,
+    const unsigned long long* __restrict__ masks, const int* __restrict__ offs,
+    const int* __restrict__ ovf, const int* __restrict__ rounds,
+    const int* __restrict__ tile_list, const int* __restrict__ seg_begin,
+    const int* __restrict__ seg_end, const int* __restrict__ count_ptr, int* flags,
+    float* __restrict__ tb) {
+    using R = float;
+    using W = Win<3, S, TSH>;
+    static_assert(S == 3, "uGIMP");
+    static_assert(kWinSlack == 0, "lane-owned bases assume a slack-free window");
+    static_assert(W::NBN == 64 && kLocalBits == 6, "64 bases per tile");
+    constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
+    constexpr int NF = MODE == 0 ? 7 : 3;
+    static_assert(kPairedP2G<3, NF, R, WN>, "paired window");
+    constexpr int kPairGroups = 15;  // pair groups over ranks 0..29 (masks hold ranks < kOwnedLast)
+    static_assert(2 * kPairGroups < kOwnedLast, "a pair group's ranks have masks");
+    constexpr int kCap = 192;        // drift list entries (a round appends at most 128)
+    __shared__ __attribute__((aligned(16))) R win_[2 * NF * WN];
+    __shared__ unsigned claim_[64];
+    __shared__ int drift_[kCap];
+    __shared__ float kc_[3 * kStencilK];
+    __shared__ int item_base_[64];
+    lds_ptr<R> win = (lds_ptr<R>)win_;
+    lds_ptr<unsigned> claim = (lds_ptr<unsigned>)claim_;
+    lds_ptr<int> drift = (lds_ptr<int>)drift_;
+    lds_ptr<float> kc = (lds_ptr<float>)kc_;
+    lds_ptr<int> item_base = (lds_ptr<int>)item_base_;
+    stage_stencil_consts(g, kc);
+    __syncthreads();
+    R* fields[NF];
+    scatter_fields<3, R, MODE, NF>(gp, fields);
+    const int lane = threadIdx.x;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    unsigned tag = unsigned(lane);
+    int flag = 0;
+    int e0, e1, est;
+    xcd_chunk(*count_ptr, e0, e1, est);
+    for (int e = e0; e < e1; e += est) {
+        const int tile = tile_list[e];
+        TileRounds tr;
+        tr.init(tile, lane, masks, offs, ovf, rounds, seg_begin, seg_end);
+        int w0[3];
+        tile_window_origin<3>(tg, tile, w0);
+        // the second window only for a tile with a round of two groups
+        const int np = min((tr.nr + 1) >> 1, kPairGroups);
+        bool two_any = false;
+        for (int k = 0; k + 1 < np; ++k)
+            two_any = two_any || __popcll(tr.tmask[2 * k]) + __popcll(tr.tmask[2 * k + 2]) <= 64;
+        using f4 = float __attribute__((ext_vector_type(4)));
+        using lds_f4 = volatile __attribute__((address_space(3))) f4*;
+        lds_f4 win4 = reinterpret_cast<lds_f4>(win);
+        constexpr int WN4 = NF * WN / 4;
+        static_assert(NF * WN % 4 == 0, "whole float4 windows");
+        for (int k = lane; k < (two_any ? 2 : 1) * WN4; k += 64) win4[k] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        int ndrift = 0;
+        // drifted particles drift[k0, k0 + n): claim rounds into window 0,
+        // global atomics for a stencil outside the window
+        auto drift_batch = [&](int k0, int n) __attribute__((always_inline)) {
+            ScatterIn<R> P;
+            Stencil<3, S, R> st;
+            bool pend = false;
+            int node0 = 0, bidx = 0;
+            if (lane < n) {
+                const int i = drift[k0 + lane];
+                load_scatter_in<3, R, MODE>(pp, i, P);
+                int fl2 = 0;
+                if (make_stencil3_lds(g, P.x, st, fl2, kc)) {
+                    if (window_slot<3, NB, WW, ZS>(st.base, w0, node0, bidx)) {
+                        pend = true;
+                    } else {
+                        for_stencil_rows<3, S, R>(st, [&](int b, int c, int a, R N, R dNx, R dNy, R dNz) {
+                            if (N != R(0)) {
+                                R cv[NF];
+                                owned_node_values<3, R, MODE>(N, dNx, dNy, dNz, P, G, cv);
+                                const long long nn = node_id(g, st.base[0] + a, st.base[1] + b, st.base[2] + c);
+#pragma unroll
+                                for (int f = 0; f < NF; ++f) atomic_add(&fields[f][nn], cv[f]);
+                            }
+                        });
+                    }
+                }
+            }
+            while (__ballot(pend) != 0ull) {
+                bool won = false;
+                if (pend) {
+                    claim[bidx] = tag;
+                    won = claim[bidx] == tag;
+                }
+                tag += 64u;
+                if (won) {
+                    for_stencil_rows<3, S, R>(st, [&](int b, int c, int a, R N, R dNx, R dNy, R dNz) {
+                        if (N != R(0)) {
+                            R cv[NF];
+                            owned_node_values<3, R, MODE>(N, dNx, dNy, dNz, P, G, cv);
+                            window_add_node<NF, R, WN, true>(win, node0 + c * ZS + b * WW + a, cv);
+                        }
+                    });
+                    pend = false;
+                }
+            }
+        };
+        auto drift_all = [&]() __attribute__((always_inline)) {
+            for (int k0 = 0; k0 < ndrift; k0 += 64) drift_batch(k0, min(64, ndrift - k0));
+            ndrift = 0;
+        };
+        // One lane's pair: particles i0 (a0) and i1 (a1) of tile-local base l,
+        // added into window side.  An inactive particle is replaced by the
+        // tile's first one with zero weights.
+        auto pair = [&](int i0, bool a0, int i1, bool a1, int l, int side) __attribute__((always_inline)) {
+            int lx, ly, lz;
+            tile_local_xyz<3>(TSH, l, lx, ly, lz);
+            const int wb = lz * ZS + ly * WW + lx;
+            lds_ptr<R> ws = win + side * NF * WN;
+            const int bx = w0[0] + lx, by = w0[1] + ly, bz = w0[2] + lz;
+            ScatterIn<R> P0, P1;
+            load_scatter_in<3, R, MODE>(pp, a0 ? i0 : tr.beg, P0);
+            load_scatter_in<3, R, MODE>(pp, a1 ? i1 : tr.beg, P1);
+            Stencil<3, S, R> s0, s1;
+            int f0 = 0, f1 = 0;
+            const bool ok0 = make_stencil3_lds(g, P0.x, s0, f0, kc);
+            const bool ok1 = make_stencil3_lds(g, P1.x, s1, f1, kc);
+            if (MODE == 0) {
+                if (a0) flag |= f0;
+                if (a1) flag |= f1;
+            }
+            const bool h0 = a0 && ok0 && s0.base[0] == bx && s0.base[1] == by && s0.base[2] == bz;
+            const bool h1 = a1 && ok1 && s1.base[0] == bx && s1.base[1] == by && s1.base[2] == bz;
+            owned_mask_stencil(s0, h0);
+            owned_mask_stencil(s1, h1);
+            if (h0 || h1) {
+                if constexpr (MODE == 0) {
+                    // per particle f_k = dNx P_k + Nx Q_k per (y, z) row, as in p2g3_window
+                    const R m0 = P0.m, m1 = P1.m, v0 = -P0.vol, v1 = -P1.vol;
+                    const R p00 = m0 * P0.v[0], p01 = m0 * P0.v[1], p02 = m0 * P0.v[2];
+                    const R p10 = m1 * P1.v[0], p11 = m1 * P1.v[1], p12 = m1 * P1.v[2];
+                    static_for<S>([&](auto c) {
+                        static_for<S>([&](auto b) {
+                            const R A0 = s0.N[1][b] * s0.N[2][c], A1 = s1.N[1][b] * s1.N[2][c];
+                            const R B0 = s0.dN[1][b] * s0.N[2][c], B1 = s1.dN[1][b] * s1.N[2][c];
+                            const R C0 = s0.N[1][b] * s0.dN[2][c], C1 = s1.N[1][b] * s1.dN[2][c];
+                            const R mA0 = m0 * A0, mA1 = m1 * A1;
+                            const R Px0 = v0 * P0.sg[0] * A0, Py0 = v0 * P0.sg[3] * A0,
+                                    Pz0 = v0 * P0.sg[5] * A0;
+                            const R Px1 = v1 * P1.sg[0] * A1, Py1 = v1 * P1.sg[3] * A1,
+                                    Pz1 = v1 * P1.sg[5] * A1;
+                            const R Qx0 = v0 * P0.sg[3] * B0 + v0 * P0.sg[5] * C0 + mA0 * G.g[0];
+                            const R Qy0 = v0 * P0.sg[1] * B0 + v0 * P0.sg[4] * C0 + mA0 * G.g[1];
+                            const R Qz0 = v0 * P0.sg[4] * B0 + v0 * P0.sg[2] * C0 + mA0 * G.g[2];
+                            const R Qx1 = v1 * P1.sg[3] * B1 + v1 * P1.sg[5] * C1 + mA1 * G.g[0];
+                            const R Qy1 = v1 * P1.sg[1] * B1 + v1 * P1.sg[4] * C1 + mA1 * G.g[1];
+                            const R Qz1 = v1 * P1.sg[4] * B1 + v1 * P1.sg[2] * C1 + mA1 * G.g[2];
+                            static_for<S>([&](auto a) {
+                                const R Nx0 = s0.N[0][a], dNx0 = s0.dN[0][a];
+                                const R Nx1 = s1.N[0][a], dNx1 = s1.dN[0][a];
+                                const R N0 = Nx0 * A0, N1 = Nx1 * A1;
+                                if (N0 != R(0) || N1 != R(0)) {
+                                    R cv[7];
+                                    cv[0] = m0 * N0 + m1 * N1;
+                                    cv[1] = p00 * N0 + p10 * N1;
+                                    cv[2] = p01 * N0 + p11 * N1;
+                                    cv[3] = p02 * N0 + p12 * N1;
+                                    cv[4] = (dNx0 * Px0 + Nx0 * Qx0) + (dNx1 * Px1 + Nx1 * Qx1);
+                                    cv[5] = (dNx0 * Py0 + Nx0 * Qy0) + (dNx1 * Py1 + Nx1 * Qy1);
+                                    cv[6] = (dNx0 * Pz0 + Nx0 * Qz0) + (dNx1 * Pz1 + Nx1 * Qz1);
+                                    window_add_node<7, R, WN, true>(
+                                        ws, wb + int(c) * ZS + int(b) * WW + int(a), cv);
+                                }
+                                __builtin_amdgcn_sched_barrier(0);
+                            });
+                        });
+                    });
+                } else {
+                    static_for<S>([&](auto c) {
+                        static_for<S>([&](auto b) {
+                            const R A0 = s0.N[1][b] * s0.N[2][c], A1 = s1.N[1][b] * s1.N[2][c];
+                            static_for<S>([&](auto a) {
+                                const R N0 = s0.N[0][a] * A0, N1 = s1.N[0][a] * A1;
+                                if (N0 != R(0) || N1 != R(0)) {
+                                    R cv[3];
+#pragma unroll
+                                    for (int d = 0; d < 3; ++d) cv[d] = P0.v[d] * N0 + P1.v[d] * N1;
+                                    window_add_node<3, R, WN, true>(
+                                        ws, wb + int(c) * ZS + int(b) * WW + int(a), cv);
+                                }
+                                __builtin_amdgcn_sched_barrier(0);
+                            });
+                        });
+                    });
+                }
+            }
+            const bool d0 = a0 && ok0 && !h0, d1 = a1 && ok1 && !h1;
+            const unsigned long long dm0 = __ballot(d0), dm1 = __ballot(d1);
+            if ((dm0 | dm1) != 0ull) {
+                if (d0) drift[ndrift + __popcll(dm0 & lt)] = i0;
+                ndrift += __popcll(dm0);
+                if (d1) drift[ndrift + __popcll(dm1 & lt)] = i1;
+                ndrift += __popcll(dm1);
+                if (ndrift > kCap - 128) drift_all();
+            }
+        };
+        // Packed rounds over the pair groups of ranks 0..29, then ranks from 30
+        // on with one lane per base; one call site of pair().  The masks and
+        // offsets are read at wave-uniform ranks and selected per lane.
+        int k = 0, rt = 2 * kPairGroups;
+        while (k < np || rt < tr.nr) {
+            int i0 = 0, i1 = 0, l = lane, side = 0;
+            bool a0 = false, a1 = false;
+            if (k < np) {
+                const int rA = 2 * k;
+                const bool hasB = k + 1 < np;
+                const unsigned long long mA = tr.tmask[rA];
+                const unsigned long long mB = hasB ? tr.tmask[rA + 2] : 0ull;
+                const unsigned long long pA = rA + 1 < tr.nr ? tr.tmask[rA + 1] : 0ull;
+                const unsigned long long pB = hasB && rA + 3 < tr.nr ? tr.tmask[rA + 3] : 0ull;
+                const int oA = tr.toff[rA], oA1 = tr.toff[rA + 1];
+                const int oB = hasB ? tr.toff[rA + 2] : 0, oB1 = hasB ? tr.toff[rA + 3] : 0;
+                const int nA = __popcll(mA), nB = __popcll(mB);
+                const bool two = hasB && nA + nB <= 64;
+                // item -> base: the base's lane writes its base at its item index
+                if ((mA >> lane) & 1ull) item_base[__popcll(mA & lt)] = lane;
+                if (two && ((mB >> lane) & 1ull)) item_base[nA + __popcll(mB & lt)] = lane;
+                a0 = lane < nA || (two && lane < nA + nB);
+                side = lane < nA ? 0 : 1;
+                if (a0) {
+                    l = item_base[lane];
+                    const unsigned long long p = side ? pB : pA;
+                    i0 = tr.beg + (side ? oB + lane - nA : oA + lane);
+                    a1 = (p >> l) & 1ull;
+                    i1 = tr.beg + (side ? oB1 : oA1) + __popcll(p & ((1ull << l) - 1ull));
+                } else {
+                    side = 0;
+                }
+                k += two ? 2 : 1;
+            } else {
+                a0 = tr.index(rt, i0);
+                a1 = rt + 1 < tr.nr && tr.index(rt + 1, i1);
+                rt += 2;
+            }
+            pair(i0, a0, i1, a1, l, side);
+        }
+        drift_all();
+        if (two_any) {
+            for (int k = lane; k < WN4; k += 64) {
+                const f4 a = win4[k], b = win4[WN4 + k];
+                win4[k] = a + b;
+            }
+        }
+        flush_window<3, WW, ZS, WN, NF, R>(g, w0, win, fields,
+                                           tb ? tb + (long long)e * part_stride<NF>() * WN : nullptr);
+    }
+    raise_flag(flags, flag);
+}
+
 // ---------------------------------------------------------------------------
 // The MUSL re-map in one pass (doublemapping1(_a)_OS! + doublemapping2_OS!,
 // utils_OS.jl:698-959): the tile's vsT, as (and vs when VS) are staged in LDS,

@@ -82,7 +82,14 @@ constexpr int kWinSlack = 0;
 // before the flush).  Measured on the 3D column: P2G #1 161 -> 150 us early in
 // the collapse (52.8 particles per group) and 386 -> 343 us at step 5000 (about
 // 30 per group, as the deposit spreads and compacts unevenly).
-constexpr int kScatterWindows = 2;
+#ifndef MPM_SCATTER_WINDOWS
+#define MPM_SCATTER_WINDOWS 2
+#endif
+constexpr int kScatterWindows = MPM_SCATTER_WINDOWS;
+// Wavefronts per SIMD the rank-round scatter is compiled for (the layered form: 3).
+#ifndef MPM_SCATTER_WAVES
+#define MPM_SCATTER_WAVES 4
+#endif
 
 struct TileGeom {
     int tsh;         // log2 of the tile edge in stencil bases
@@ -504,7 +511,7 @@ __device__ __forceinline__ void flush_layers(const GridGeom& g, const int* w0, l
 // goes through memory.
 // ---------------------------------------------------------------------------
 template <int DIM, int S, int TSH, typename R, int MODE, bool LAY = false>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LAY ? 3 : 4))) void k_scatter_tiled(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LAY ? 3 : MPM_SCATTER_WAVES))) void k_scatter_tiled(
     GridGeom g, TileGeom tg, PartPtrs<R> pp, GridPtrs<R> gp, Gravity<R> G,
     const unsigned* __restrict__ keys, const int* __restrict__ tile_list,
     const int* __restrict__ seg_begin, const int* __restrict__ seg_end,

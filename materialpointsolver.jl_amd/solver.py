@@ -215,8 +215,9 @@ class HIPModel:
 
     def set_option(self, option, value):
         """Runtime option: "sort_interval" (>= 1), "tiles" (bool), "scatter"
-        (P2G #1's tile scatter form: "rounds", "rounds_layered", "owned" or
-        "owned_layers"), "remap_scatter" (the re-map momentum scatter's form,
+        (P2G #1's tile scatter form: "rounds", "rounds_layered", "owned",
+        "owned_layers" or "pairs"; None = automatic: "pairs" in FP32 3D uGIMP,
+        else "rounds"), "remap_scatter" (the re-map momentum scatter's form,
         None = automatic: "owned" in FP32 3D uGIMP, else as "scatter"),
         "fused_remap" (bool: the one-pass MUSL re-map; off by default),
         "tile_partials" (bool: tile windows stored and summed by the grid solves
@@ -234,7 +235,7 @@ class HIPModel:
             self._check(self.lib.mpm_set_option(self.ctx, capi.MPM_OPT_PROFILE_MASK, mask))
             return
         if option in ("scatter", "remap_scatter"):
-            if option == "remap_scatter" and value is None:
+            if value is None:
                 code = -1  # automatic
             elif value in capi.SCATTER:
                 code = capi.SCATTER[value]

@@ -147,7 +147,8 @@ def test_remap_one_pass_and_split(mpm, orc, fused, scatter):
     _compare(mpm, orc, _c3(mpm, basis="linear", scheme="USL"), 60, 1e-9, sort_interval=5, **kw)
 
 
-@pytest.mark.parametrize("scatter", ["owned", "owned_layers", "rounds", "rounds_layered"])
+@pytest.mark.parametrize("scatter",
+                         ["owned", "owned_layers", "rounds", "rounds_layered", "pairs"])
 def test_fp32_scatter_forms(mpm, orc, scatter):
     tol = dict(xi=2e-4, vs=5e-4, sigma=2e-3, eps_s=5e-4, eps_q=2e-3, omega=1e-5)
     _compare(mpm, orc, _c3(mpm, eps="FP32", constitutive="druckerprager"), 100, tol,
@@ -158,7 +159,8 @@ def test_fp32_scatter_forms(mpm, orc, scatter):
 
 # The re-map's momentum scatter in each form, with P2G #1 in the rank rounds
 # (FP32 3D uGIMP takes the base-owned form by default), sorted often and rarely
-@pytest.mark.parametrize("remap_scatter", ["rounds", "owned", "owned_layers", "rounds_layered"])
+@pytest.mark.parametrize("remap_scatter",
+                         ["rounds", "owned", "owned_layers", "rounds_layered", "pairs"])
 def test_fp32_remap_scatter_forms(mpm, orc, remap_scatter):
     tol = dict(xi=2e-4, vs=5e-4, sigma=2e-3, eps_s=5e-4, eps_q=2e-3, omega=1e-5)
     for sort in (10, 400):
@@ -178,6 +180,62 @@ def test_fp32_rounds_layered(mpm, orc, partials):
         for va in ("a", "v"):
             _compare(mpm, orc, _c3(mpm, eps="FP32", constitutive="druckerprager", va=va), 100,
                      tol, sort_interval=sort, scatter="rounds_layered", tile_partials=partials)
+
+
+# The pair rounds (FP32 3D uGIMP) with tile partials and with the atomic
+# flush, sorted often and rarely (drift off the sort-time bases and out of the
+# windows), P2G #1 and the re-map scatter, :a and :v
+@pytest.mark.parametrize("partials", [True, False])
+def test_fp32_pairs(mpm, orc, partials):
+    tol = dict(xi=2e-4, vs=5e-4, sigma=2e-3, eps_s=5e-4, eps_q=2e-3, omega=1e-5)
+    for sort in (5, 400):
+        for va in ("a", "v"):
+            _compare(mpm, orc, _c3(mpm, eps="FP32", constitutive="druckerprager", va=va), 100,
+                     tol, sort_interval=sort, scatter="pairs", remap_scatter="pairs",
+                     tile_partials=partials)
+
+
+# Pair rounds on dual cells of 64 particles (4x4x4 per cell): pair groups up
+# to rank 29, then ranks 30..63 one lane per base (rank 31 on in the merged
+# last group of the sort), against the plain global-atomic kernels, which sum
+# the same terms in another order.
+def test_fp32_pairs_dense_cells(mpm):
+    args, grid, mp, attr, bc = mpm.scenes.column_collapse_3d(
+        h=0.02, width=0.08, height=0.04, box_xy=0.24, box_z=0.08, eps="FP32", nsteps=10)
+    lp = 0.005
+    w2 = 0.04
+    lat = lambda lo, hi: np.arange(lo + lp / 2, hi, lp)
+    pts = mpm.types.meshbuilder(lat(-w2, w2), lat(-w2, w2), lat(0.0, 0.04))
+    mp = mpm.types.UserParticle3D(ϵ="FP32", NIC=27, dx=lp, dy=lp, dz=lp, ξ=pts,
+                                  ρs=np.full(len(pts), mpm.scenes.RHO))
+    attr = mpm.types.UserProperty(ϵ="FP32", nid=np.ones(mp.np), ν=[mpm.scenes.NU],
+                                  Es=[mpm.scenes.E], Gs=[mpm.scenes.GS], Ks=[mpm.scenes.KS],
+                                  σt=[0.0], ϕ=[np.radians(mpm.scenes.PHI_DEG)], ϕr=[0.0],
+                                  ψ=[0.0], c=[0.0], cr=[0.0], Hp=[0.0])
+    dt = float(args.ΔT)
+    x0 = np.asarray(mp.ξ, dtype=np.float64)
+    tiled = mpm.HIPModel(grid, mp, attr, bc, sort_interval=10)
+    plain = mpm.HIPModel(grid, mp, attr, bc, sort_interval=10)
+    try:
+        tiled.set_option("scatter", "pairs")
+        tiled.set_option("remap_scatter", "pairs")
+        plain.set_option("tiles", False)
+        sa = tiled.step_args(args)
+        T = 0.0
+        for nsteps, tol in ((1, 1e-5), (9, 1e-4)):
+            tiled.run(sa, dt, T, nsteps)
+            T = plain.run(sa, dt, T, nsteps)
+            a, b = tiled.download(), plain.download()
+            for k in ("xi", "vs", "sigma"):
+                ga = a[k] - x0 if k == "xi" else a[k]
+                gb = b[k] - x0 if k == "xi" else b[k]
+                r = _rel(ga, gb)
+                print(f"  pairs, 64 per cell, after {nsteps} more step(s): {k} rel L2 {r:.2e}",
+                      flush=True)
+                assert r <= tol, (k, nsteps, r)
+    finally:
+        tiled.close()
+        plain.close()
 
 
 # Mohr-Coulomb softening.  2D: cohesion max(c + Hp ϵq, cr) (mohrcoulomb.jl:36;

@@ -132,7 +132,7 @@ def test_autotune_picks_the_fastest_and_leaves_it_set(mpm):
         def run(self, sargs, dt, Ti, n, graph=False):
             cost = {5: 3.0, 10: 1.0, 20: 1.5, 40: 2.0}[self.opts["sort_interval"]]
             form = {"owned": 1.0, "owned_layers": 0.9, "rounds": 1.3,
-                    "rounds_layered": 1.1}[self.opts.get("scatter", "owned")]
+                    "rounds_layered": 1.1, "pairs": 0.95}[self.opts.get("scatter", "owned")]
             self.now += n * cost * (form if self.opts["tiles"] else 4.0)
             return Ti + n * dt
 
@@ -146,14 +146,14 @@ def test_autotune_picks_the_fastest_and_leaves_it_set(mpm):
                       "remap_scatter": None}
     assert len(table) == 8 and abs(min(t[3] for t in table) - 1.3) < 1e-12
     assert Ti == 8 * 5 * 0.5  # every trial advanced the clock
-    # with every scatter form: 4 forms x 4 intervals with tiles + 4 without
+    # with every scatter form: 5 forms x 4 intervals with tiles + 4 without
     m = FakeModel()
     best, table, Ti = mpm.autotune(m, None, 0.5, 0.0, steps=4, warm=1, clock=lambda: m.now,
                                    scatters=mpm.SCATTER_FORMS)
     assert best == dict(sort_interval=10, tiles=True, scatter="owned_layers")
     assert m.opts == {"sort_interval": 10, "tiles": True, "scatter": "owned_layers",
                       "remap_scatter": None}
-    assert len(table) == 20 and abs(min(t[3] for t in table) - 0.9) < 1e-12
+    assert len(table) == 24 and abs(min(t[3] for t in table) - 0.9) < 1e-12
     # a (P2G #1, re-map scatter) pair sets both forms
     m = FakeModel()
     best, table, Ti = mpm.autotune(m, None, 0.5, 0.0, steps=4, warm=1, clock=lambda: m.now,
