@@ -842,9 +842,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODE == 0 ? 
                                     cv[1] = p00 * N0 + p10 * N1;
                                     cv[2] = p01 * N0 + p11 * N1;
                                     cv[3] = p02 * N0 + p12 * N1;
-                                    cv[4] = (dNx0 * Px0 + Nx0 * Qx0) + (dNx1 * Px1 + Nx1 * Qx1);
-                                    cv[5] = (dNx0 * Py0 + Nx0 * Qy0) + (dNx1 * Py1 + Nx1 * Qy1);
-                                    cv[6] = (dNx0 * Pz0 + Nx0 * Qz0) + (dNx1 * Pz1 + Nx1 * Qz1);
+                                    cv[4] = fmaf(Nx1, Qx1, fmaf(dNx1, Px1, fmaf(Nx0, Qx0, dNx0 * Px0)));
+                                    cv[5] = fmaf(Nx1, Qy1, fmaf(dNx1, Py1, fmaf(Nx0, Qy0, dNx0 * Py0)));
+                                    cv[6] = fmaf(Nx1, Qz1, fmaf(dNx1, Pz1, fmaf(Nx0, Qz0, dNx0 * Pz0)));
                                     window_add_node<7, R, WN, true>(
                                         ws, wb + int(c) * ZS + int(b) * WW + int(a), cv);
                                 }
