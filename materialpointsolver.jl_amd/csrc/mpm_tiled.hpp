@@ -174,6 +174,39 @@ __device__ __forceinline__ void stage_window(const GridGeom& g, const int* w0, R
     }
 }
 
+// Staged windows of the tile gathers (G2P #2, the re-map gather): node
+// (x, y, z) of the window at x + SY y + SZ z.  With the dense strides (WW,
+// WW^2) the 4x4x4 tile's bases in lane order (tile_local) put two lanes of
+// every 32 on one LDS bank at each stencil slot; FP32 3D uGIMP pads the rows
+// to 16 and the layers to 100 nodes, which puts the 32 lanes of each half and
+// the 64 lanes of a full rank group on distinct banks at all 27 slots (586
+// nodes per field instead of 216).
+template <int DIM, int S, int TSH, typename R>
+struct GWin {
+    using W = Win<DIM, S, TSH>;
+    static constexpr bool PAD = DIM == 3 && S == 3 && TSH == 2 && sizeof(R) == 4;
+    static constexpr int SY = PAD ? 16 : W::WW;
+    static constexpr int SZ = PAD ? 100 : W::ZS;
+    static constexpr int N = PAD ? (W::WW - 1) * (1 + SY + SZ) + 1 : W::WN;  // nodes per field
+};
+
+// stage_window into a GWin layout (only the window's nodes are written).
+template <int DIM, int WW, int ZS, int WN, int SY, int SZ, int GN, int NF, typename R>
+__device__ __forceinline__ void stage_window_strided(const GridGeom& g, const int* w0,
+                                                     R* const* fields, R* lds) {
+    for (int k = threadIdx.x; k < WN; k += blockDim.x) {
+        int kx, ky, kz;
+        if (!window_node<DIM, WW, ZS>(k, kx, ky, kz)) continue;
+        const int gx = w0[0] + kx, gy = w0[1] + ky, gz = DIM == 3 ? w0[2] + kz : 0;
+        const bool in = gx >= 0 && gx < g.nn[0] && gy >= 0 && gy < g.nn[1] &&
+                        (DIM == 2 || (gz >= 0 && gz < g.nn[2]));
+        const long long n = in ? node_id(g, gx, gy, gz) : 0;
+        const int p = kx + SY * ky + SZ * kz;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) lds[f * GN + p] = in ? fields[f][n] : R(0);
+    }
+}
+
 // FP32 3D P2G #1 windows interleave field pairs: (m, px), (py, pz), (fx, fy) as
 // 64-bit pairs per node and fz alone, so a stencil slot reads and writes three
 // pairs and one value (4 + 4 LDS instructions instead of 7 + 7).  Field f of
@@ -834,9 +867,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_wave
     StepArgs<R> sa, const int* __restrict__ tile_list, const int* __restrict__ seg_begin,
     const int* __restrict__ seg_end, const int* __restrict__ count_ptr, int* flags) {
     using W = Win<DIM, S, TSH>;
+    using GW = GWin<DIM, S, TSH, R>;
     constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
+    constexpr int SY = GW::SY, SZ = GW::SZ, GN = GW::N;
     constexpr int NF = DIM * DIM;
-    __shared__ R uwin[DIM * WN];
+    __shared__ R uwin[DIM * GN];
     R* ufields[3] = {gp.du[0], gp.du[1], DIM == 3 ? gp.du[2] : gp.du[1]};
     int flag = 0;
     int e0, e1, est;
@@ -847,7 +882,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_wave
         int w0[3];
         tile_window_origin<DIM>(tg, tile, w0);
         __syncthreads();
-        stage_window<DIM, WW, ZS, WN, DIM, R>(g, w0, ufields, uwin);
+        stage_window_strided<DIM, WW, ZS, WN, SY, SZ, GN, DIM, R>(g, w0, ufields, uwin);
         __syncthreads();
         for (int i = beg + int(threadIdx.x); i < end; i += blockDim.x) {
             double x[3];
@@ -860,7 +895,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_wave
                 continue;
             }
             int wb, bidx;
-            const bool inside = window_slot<DIM, NB, WW, ZS>(st.base, w0, wb, bidx);
+            const bool inside = window_slot<DIM, NB, SY, SZ>(st.base, w0, wb, bidx);
             R F[NF];
 #pragma unroll
             for (int k = 0; k < NF; ++k) F[k] = R(0);
@@ -879,11 +914,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_wave
             if (inside) {
                 // branch-free: a zero weight comes with zero gradients for uGIMP;
                 // the linear basis keeps the test (its gradient is not zero there)
-                for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R dNx, R dNy, R dNz) {
+                for_stencil_local<DIM, S, R>(st, SY, SZ, [&](int off, R N, R dNx, R dNy, R dNz) {
                     if (S == 3 || N != R(0)) {
                         R u[3];
 #pragma unroll
-                        for (int d = 0; d < DIM; ++d) u[d] = uwin[d * WN + wb + off];
+                        for (int d = 0; d < DIM; ++d) u[d] = uwin[d * GN + wb + off];
                         acc(u, dNx, dNy, dNz);
                     }
                 });
@@ -920,9 +955,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_wave
                                                    const int* __restrict__ seg_end,
                                                    const int* __restrict__ count_ptr) {
     using W = Win<DIM, S, TSH>;
+    using GW = GWin<DIM, S, TSH, R>;
     constexpr int WW = W::WW, WN = W::WN, NB = W::NB, ZS = W::ZS;
+    constexpr int SY = GW::SY, SZ = GW::SZ, GN = GW::N;
     constexpr int NG = (VS ? 3 : 2) * DIM;  // gathered: vsT, as[, vs (or vsT again, see pic_field)]
-    __shared__ R gwin[NG * WN];
+    __shared__ R gwin[NG * GN];
     R* const* vsf = pic_field(gp, sa);
     R* gfields[3 * DIM];
 #pragma unroll
@@ -939,7 +976,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_wave
         int w0[3];
         tile_window_origin<DIM>(tg, tile, w0);
         __syncthreads();  // the previous tile's reads of gwin are done
-        stage_window<DIM, WW, ZS, WN, NG, R>(g, w0, gfields, gwin);
+        stage_window_strided<DIM, WW, ZS, WN, SY, SZ, GN, NG, R>(g, w0, gfields, gwin);
         __syncthreads();
         for (int i = beg + int(threadIdx.x); i < end; i += blockDim.x) {
             double x[3];
@@ -959,18 +996,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gather_wave
                 continue;
             }
             int wb, bidx;
-            const bool inside = window_slot<DIM, NB, WW, ZS>(st.base, w0, wb, bidx);
+            const bool inside = window_slot<DIM, NB, SY, SZ>(st.base, w0, wb, bidx);
             R A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, C[3] = {0, 0, 0};
             if (inside) {
                 // branch-free: zero weights add exact zeros (staged values are finite)
-                for_stencil_local<DIM, S, R>(st, WW, ZS, [&](int off, R N, R, R, R) {
+                for_stencil_local<DIM, S, R>(st, SY, SZ, [&](int off, R N, R, R, R) {
                     const R* L = gwin + wb + off;
                     R vT[3], vA[3], vS[3];
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) {
-                        vT[d] = L[d * WN];
-                        vA[d] = L[(DIM + d) * WN];
-                        vS[d] = VS ? L[(2 * DIM + d) * WN] : R(0);
+                        vT[d] = L[d * GN];
+                        vA[d] = L[(DIM + d) * GN];
+                        vS[d] = VS ? L[(2 * DIM + d) * GN] : R(0);
                     }
                     g2p1_accumulate<DIM, R>(sa.va, N, vT, vA, vS, A, B, C);
                 });
