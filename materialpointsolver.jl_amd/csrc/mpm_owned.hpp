@@ -30,6 +30,10 @@
 // momentum in one pass over the tile's particles and the force in a second
 // (FP32), and FP64 3D also takes the z layers of the stencil one at a time
 // (OwnedPasses).
+//
+// The file also holds the pair rounds (k_scatter_pairs, FP32 3D uGIMP P2G #1
+// by default): between the two, a lane holds two particles of one base and
+// adds both per stencil slot with one window read-add-write.
 #pragma once
 #include <hip/hip_runtime.h>
 
